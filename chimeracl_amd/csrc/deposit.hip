@@ -1,0 +1,524 @@
+// Particle <-> grid kernels: deterministic CIC deposition of rho and J into
+// the azimuthal modes, axis post-processing, ghost-row warp, and the field
+// gather + relativistic Boris push.  gfx950, wave64.
+//
+// Deposition (replaces the 4-colour depose_scalar / depose_vector launches,
+// grid_deposit_m1.cl:20-327, grid_methods_cl.py:44-95) runs in two passes and
+// uses no float atomics:
+//   pass 1 (one lane per valid cell): the cell's particles, in sorted order,
+//          are summed into private accumulators exactly as the reference
+//          kernel does; the 4 node corners x components are written to a
+//          cell-major staging buffer (coalesced, component-major).
+//   pass 2 (one lane per node): each node adds the partial sums of its (up
+//          to) four cells in the reference's colour order 0,1,2,3 -- the order
+//          in which the reference's serialised colour launches add them.
+// The result is bit-identical to the reference's deposit for a given
+// in-cell order.
+#include "chm_common.h"
+
+template <typename T> __device__ __forceinline__ T dsqrt(T v);
+template <> __device__ __forceinline__ double dsqrt<double>(double v) { return sqrt(v); }
+template <> __device__ __forceinline__ float dsqrt<float>(float v) { return sqrtf(v); }
+template <typename T> __device__ __forceinline__ T dfloor(T v);
+template <> __device__ __forceinline__ double dfloor<double>(double v) { return floor(v); }
+template <> __device__ __forceinline__ float dfloor<float>(float v) { return floorf(v); }
+
+struct PlanePtrs {
+  void *m0[3];
+  void *mc[3][CHM_MAXM];
+};
+
+// ------------------------------------------------------------------ pass 1
+template <typename T, int M, bool VEC>
+__global__ void __launch_bounds__(256)
+    k_cell_sums(const uint32_t *__restrict__ sort_indx, const uint32_t *__restrict__ cell_offset,
+                const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+                const T *__restrict__ ux, const T *__restrict__ uy, const T *__restrict__ uz,
+                const T *__restrict__ g_inv, const T *__restrict__ w, T charge, int Nx, int Nr,
+                T xmin, T dx_inv, T rmin, T dr_inv, T *__restrict__ part) {
+  constexpr int NK = VEC ? 3 : 1;
+  const int Nx_cell = Nx - 1, Nr_cell = Nr - 1;
+  const int64_t ncell = (int64_t)Nx_cell * Nr_cell;
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int ir = (int)(c / Nx_cell), ix = (int)(c - (int64_t)ir * Nx_cell);
+  if (!(ix > 0 && ix < Nx_cell - 1 && ir < Nr_cell - 1)) return;
+  uint32_t ip0 = cell_offset[c], ip1 = cell_offset[c + 1];
+  if (ip0 == ip1) return;
+  T a0[NK][2][2];
+  T ar[NK][M > 0 ? M : 1][2][2], ai[NK][M > 0 ? M : 1][2][2];
+#pragma unroll
+  for (int k = 0; k < NK; k++)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        a0[k][i][j] = 0;
+#pragma unroll
+        for (int m = 0; m < (M > 0 ? M : 1); m++) ar[k][m][i][j] = ai[k][m][i][j] = 0;
+      }
+  for (uint32_t ip = ip0; ip < ip1; ip++) {
+    uint32_t p = sort_indx[ip];
+    T xp = x[p], yp = y[p], zp = z[p];
+    T jp[3];
+    T wp;
+    if (VEC) {
+      jp[0] = ux[p];
+      jp[1] = uy[p];
+      jp[2] = uz[p];
+      wp = w[p] * g_inv[p] * charge;
+    } else {
+      wp = w[p] * charge;
+    }
+    T rp = dsqrt<T>(yp * yp + zp * zp);
+    T rp_inv;
+    if (VEC) {
+      rp_inv = 0;
+      if (rp > 0) rp_inv = (T)1. / rp;
+    } else {
+      rp_inv = (T)1. / rp;
+    }
+    // ((y + i z)/r)^m, m = 1..M
+    T er[M > 0 ? M + 1 : 2], ei[M > 0 ? M + 1 : 2];
+    er[1] = yp * rp_inv;
+    ei[1] = zp * rp_inv;
+#pragma unroll
+    for (int m = 2; m <= M; m++) {
+      er[m] = er[m - 1] * er[1] - ei[m - 1] * ei[1];
+      ei[m] = er[m - 1] * ei[1] + ei[m - 1] * er[1];
+    }
+    T sX1 = (xp - xmin) * dx_inv - (T)ix;
+    T sX0 = (T)1.0 - sX1;
+    T sR1 = (rp - rmin) * dr_inv - (T)ir;
+    T sR0 = (T)1.0 - sR1;
+    sX0 *= wp;
+    sX1 *= wp;
+    T C[2][2];
+    C[0][0] = sR0 * sX0;
+    C[0][1] = sR0 * sX1;
+    C[1][0] = sR1 * sX0;
+    C[1][1] = sR1 * sX1;
+#pragma unroll
+    for (int k = 0; k < NK; k++)
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          if (VEC) {
+            T jproj = C[i][j] * jp[k];
+            a0[k][i][j] += C[i][j] * jp[k];
+#pragma unroll
+            for (int m = 1; m <= M; m++) {
+              ar[k][m - 1][i][j] += jproj * er[m];
+              ai[k][m - 1][i][j] += jproj * ei[m];
+            }
+          } else {
+            a0[k][i][j] += C[i][j];
+#pragma unroll
+            for (int m = 1; m <= M; m++) {
+              ar[k][m - 1][i][j] += C[i][j] * er[m];
+              ai[k][m - 1][i][j] += C[i][j] * ei[m];
+            }
+          }
+        }
+  }
+  // staging layout: part[((k*(1+2M) + q)*4 + corner) * ncell + c]
+  constexpr int NQ = 1 + 2 * M;
+#pragma unroll
+  for (int k = 0; k < NK; k++)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        int corner = 2 * i + j;
+        part[((int64_t)(k * NQ) * 4 + corner) * ncell + c] = a0[k][i][j];
+#pragma unroll
+        for (int m = 1; m <= M; m++) {
+          part[((int64_t)(k * NQ + 2 * m - 1) * 4 + corner) * ncell + c] = ar[k][m - 1][i][j];
+          part[((int64_t)(k * NQ + 2 * m) * 4 + corner) * ncell + c] = ai[k][m - 1][i][j];
+        }
+      }
+}
+
+// ------------------------------------------------------------------ pass 2
+template <typename T, int M, int NK>
+__global__ void __launch_bounds__(256)
+    k_node_add(const uint32_t *__restrict__ cell_offset, const T *__restrict__ part, int Nx, int Nr,
+               PlanePtrs planes) {
+  const int Nx_cell = Nx - 1, Nr_cell = Nr - 1;
+  const int64_t ncell = (int64_t)Nx_cell * Nr_cell;
+  int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= (int64_t)(Nr - 1) * Nx) return;
+  int ir = (int)(node / Nx), ix = (int)(node - (int64_t)ir * Nx);
+  constexpr int NQ = 1 + 2 * M;
+  T acc[NK][NQ];
+#pragma unroll
+  for (int k = 0; k < NK; k++) {
+    acc[k][0] = ((const T *)planes.m0[k])[node];
+#pragma unroll
+    for (int m = 1; m <= M; m++) {
+      const T *pc = (const T *)planes.mc[k][m - 1];
+      acc[k][2 * m - 1] = pc[2 * node];
+      acc[k][2 * m] = pc[2 * node + 1];
+    }
+  }
+  bool any = false;
+#pragma unroll
+  for (int col = 0; col < 4; col++) {
+    int px = col & 1, pr = col >> 1;
+    int cx = ((ix & 1) == px) ? ix : ix - 1;
+    int cr = ((ir & 1) == pr) ? ir : ir - 1;
+    if (!(cx > 0 && cx < Nx_cell - 1 && cr >= 0 && cr < Nr_cell - 1)) continue;
+    int64_t c = cx + (int64_t)cr * Nx_cell;
+    if (cell_offset[c] == cell_offset[c + 1]) continue;
+    int corner = 2 * (ir - cr) + (ix - cx);
+    any = true;
+#pragma unroll
+    for (int k = 0; k < NK; k++)
+#pragma unroll
+      for (int q = 0; q < NQ; q++) acc[k][q] = acc[k][q] + part[((int64_t)(k * NQ + q) * 4 + corner) * ncell + c];
+  }
+  if (!any) return;
+#pragma unroll
+  for (int k = 0; k < NK; k++) {
+    ((T *)planes.m0[k])[node] = acc[k][0];
+#pragma unroll
+    for (int m = 1; m <= M; m++) {
+      T *pc = (T *)planes.mc[k][m - 1];
+      pc[2 * node] = acc[k][2 * m - 1];
+      pc[2 * node + 1] = acc[k][2 * m];
+    }
+  }
+}
+
+extern "C" size_t chm_deposit_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M, int32_t vector) {
+  size_t ncell = (size_t)(Nx - 1) * (Nr - 1);
+  size_t nq = (size_t)(vector ? 3 : 1) * (1 + 2 * M) * 4;
+  return chm_align(nq * ncell * sizeof(double));
+}
+
+template <typename T, int M, bool VEC>
+static void launch_deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const T *x,
+                           const T *y, const T *z, const T *ux, const T *uy, const T *uz,
+                           const T *g_inv, const T *w, T charge, const chm_grid_t *g,
+                           const PlanePtrs &pp, T *part, hipStream_t st) {
+  int64_t ncell = (int64_t)(g->Nx - 1) * (g->Nr - 1);
+  k_cell_sums<T, M, VEC><<<chm_blocks(ncell, 256), 256, 0, st>>>(
+      sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr, (T)g->xmin,
+      (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, part);
+  int64_t nnode = (int64_t)(g->Nr - 1) * g->Nx;
+  k_node_add<T, M, VEC ? 3 : 1><<<chm_blocks(nnode, 256), 256, 0, st>>>(cell_offset, part, g->Nx,
+                                                                        g->Nr, pp);
+}
+
+template <typename T, bool VEC>
+static int deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const T *x, const T *y,
+                   const T *z, const T *ux, const T *uy, const T *uz, const T *g_inv, const T *w,
+                   int32_t charge, const chm_grid_t *g, const chm_planes_t *flds, void *ws,
+                   size_t ws_bytes, void *stream) {
+  if (!g || !flds || g->M < 0 || g->M > CHM_MAXM || g->Nx < 4 || g->Nr < 3) return CHM_EINVAL;
+  if (chm_deposit_workspace_bytes(g->Nx, g->Nr, g->M, VEC) > ws_bytes || !ws) return CHM_EWORKSPACE;
+  PlanePtrs pp = {};
+  for (int k = 0; k < (VEC ? 3 : 1); k++) {
+    pp.m0[k] = flds[k].m0;
+    for (int m = 0; m < g->M; m++) pp.mc[k][m] = flds[k].mc[m];
+  }
+  // int8 charge, as the reference passes it (grid_methods_cl.py:64, 89)
+  T q = (T)(int8_t)charge;
+  hipStream_t st = (hipStream_t)stream;
+  T *part = (T *)ws;
+  switch (g->M) {
+#define CASE(MM)                                                                                   \
+  case MM:                                                                                         \
+    launch_deposit<T, MM, VEC>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, q, g, pp,    \
+                               part, st);                                                          \
+    break;
+    CASE(0) CASE(1) CASE(2) CASE(3) CASE(4)
+#undef CASE
+  }
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_deposit_scalar_f64(const uint32_t *sort_indx, const uint32_t *cell_offset,
+                                      const double *x, const double *y, const double *z,
+                                      const double *w, int32_t charge, const chm_grid_t *g,
+                                      const chm_planes_t *rho, void *ws, size_t ws_bytes,
+                                      void *stream) {
+  return deposit<double, false>(sort_indx, cell_offset, x, y, z, nullptr, nullptr, nullptr, nullptr,
+                                w, charge, g, rho, ws, ws_bytes, stream);
+}
+extern "C" int chm_deposit_scalar_f32(const uint32_t *sort_indx, const uint32_t *cell_offset,
+                                      const float *x, const float *y, const float *z,
+                                      const float *w, int32_t charge, const chm_grid_t *g,
+                                      const chm_planes_t *rho, void *ws, size_t ws_bytes,
+                                      void *stream) {
+  return deposit<float, false>(sort_indx, cell_offset, x, y, z, nullptr, nullptr, nullptr, nullptr,
+                               w, charge, g, rho, ws, ws_bytes, stream);
+}
+extern "C" int chm_deposit_vector_f64(const uint32_t *sort_indx, const uint32_t *cell_offset,
+                                      const double *x, const double *y, const double *z,
+                                      const double *ux, const double *uy, const double *uz,
+                                      const double *g_inv, const double *w, int32_t charge,
+                                      const chm_grid_t *g, const chm_planes_t *J, void *ws,
+                                      size_t ws_bytes, void *stream) {
+  return deposit<double, true>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g, J,
+                               ws, ws_bytes, stream);
+}
+extern "C" int chm_deposit_vector_f32(const uint32_t *sort_indx, const uint32_t *cell_offset,
+                                      const float *x, const float *y, const float *z,
+                                      const float *ux, const float *uy, const float *uz,
+                                      const float *g_inv, const float *w, int32_t charge,
+                                      const chm_grid_t *g, const chm_planes_t *J, void *ws,
+                                      size_t ws_bytes, void *stream) {
+  return deposit<float, true>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g, J,
+                              ws, ws_bytes, stream);
+}
+
+// ------------------------------------------------------------------ post-processing
+#define MAX_PLANES 64
+struct PlaneList {
+  void *p[MAX_PLANES];
+  int cplx[MAX_PLANES];
+};
+
+// treat_axis (row1 -= row0) then *= dV_inv[ir] (grid_generic.cl:4-60); the
+// lane of row 1 also rescales row 0 so that row 0 is read before it changes.
+template <typename T>
+__global__ void k_postproc(PlaneList pl, const double *__restrict__ dV_inv, int Nx, int Nr) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int pi = blockIdx.y;
+  int64_t n = (int64_t)(Nr - 1) * Nx;
+  if (e >= n) return;
+  int ir = 1 + (int)(e / Nx), ix = (int)(e % Nx);
+  T *a = (T *)pl.p[pi];
+  int nc = pl.cplx[pi] ? 2 : 1;
+  T dvi = (T)dV_inv[ir];
+  for (int h = 0; h < nc; h++) {
+    int64_t i1 = ((int64_t)ir * Nx + ix) * nc + h;
+    if (ir == 1) {
+      int64_t i0 = (int64_t)ix * nc + h;
+      T a0 = a[i0];
+      T v = a[i1] - a0;
+      a[i1] = v * dvi;
+      a[i0] = a0 * (T)dV_inv[0];
+    } else {
+      a[i1] = a[i1] * dvi;
+    }
+  }
+}
+
+// warp_axis_m0_d / warp_axis_m1plus_c, grid_generic.cl:63-86
+template <typename T> __global__ void k_warp(PlaneList pl, int Nx) {
+  int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  int pi = blockIdx.y;
+  if (ix >= Nx) return;
+  T *a = (T *)pl.p[pi];
+  if (pl.cplx[pi]) {
+    a[2 * ix] = -a[2 * (Nx + ix)];
+    a[2 * ix + 1] = -a[2 * (Nx + ix) + 1];
+  } else {
+    a[ix] = a[Nx + ix];
+  }
+}
+
+static int fill_plane_list(PlaneList *pl, const chm_planes_t *flds, int nflds, int M) {
+  int n = 0;
+  for (int f = 0; f < nflds; f++) {
+    if (n >= MAX_PLANES) return -1;
+    pl->p[n] = flds[f].m0;
+    pl->cplx[n++] = 0;
+    for (int m = 0; m < M; m++) {
+      if (n >= MAX_PLANES) return -1;
+      pl->p[n] = flds[f].mc[m];
+      pl->cplx[n++] = 1;
+    }
+  }
+  return n;
+}
+
+template <typename T>
+static int postproc(const chm_planes_t *flds, int32_t nflds, const double *dV_inv, int32_t Nx,
+                    int32_t Nr, int32_t M, void *stream) {
+  if (!flds || nflds <= 0 || M < 0 || M > CHM_MAXM || Nr < 2) return CHM_EINVAL;
+  PlaneList pl;
+  int np = fill_plane_list(&pl, flds, nflds, M);
+  if (np < 0) return CHM_EINVAL;
+  int64_t n = (int64_t)(Nr - 1) * Nx;
+  dim3 grid(chm_blocks(n, 256), np);
+  k_postproc<T><<<grid, 256, 0, (hipStream_t)stream>>>(pl, dV_inv, Nx, Nr);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+template <typename T>
+static int warp(const chm_planes_t *flds, int32_t nflds, int32_t Nx, int32_t Nr, int32_t M,
+                void *stream) {
+  if (!flds || nflds <= 0 || M < 0 || M > CHM_MAXM || Nr < 2) return CHM_EINVAL;
+  PlaneList pl;
+  int np = fill_plane_list(&pl, flds, nflds, M);
+  if (np < 0) return CHM_EINVAL;
+  dim3 grid(chm_blocks(Nx, 256), np);
+  k_warp<T><<<grid, 256, 0, (hipStream_t)stream>>>(pl, Nx);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_postproc_deposit_f64(const chm_planes_t *flds, int32_t nflds,
+                                        const double *dV_inv, int32_t Nx, int32_t Nr, int32_t M,
+                                        void *stream) {
+  return postproc<double>(flds, nflds, dV_inv, Nx, Nr, M, stream);
+}
+extern "C" int chm_postproc_deposit_f32(const chm_planes_t *flds, int32_t nflds,
+                                        const double *dV_inv, int32_t Nx, int32_t Nr, int32_t M,
+                                        void *stream) {
+  return postproc<float>(flds, nflds, dV_inv, Nx, Nr, M, stream);
+}
+extern "C" int chm_warp_axis_f64(const chm_planes_t *flds, int32_t nflds, int32_t Nx, int32_t Nr,
+                                 int32_t M, void *stream) {
+  return warp<double>(flds, nflds, Nx, Nr, M, stream);
+}
+extern "C" int chm_warp_axis_f32(const chm_planes_t *flds, int32_t nflds, int32_t Nx, int32_t Nr,
+                                 int32_t M, void *stream) {
+  return warp<float>(flds, nflds, Nx, Nr, M, stream);
+}
+
+// ------------------------------------------------------------------ gather + push
+struct FieldPtrs {
+  const void *e0[3], *b0[3];
+  const void *ec[3][CHM_MAXM], *bc[3][CHM_MAXM];
+};
+
+// gather_and_push, grid_deposit_m1.cl:330-511 (m0: grid_deposit_m0.cl:280-427).
+// One lane per particle index p; the reference's per-slot launch visits the
+// same set {p < Np_stay, p in a valid cell} and each particle's update only
+// reads its own data, so the results are identical.
+template <typename T, int M>
+__global__ void __launch_bounds__(256)
+    k_gather_push(const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+                  T *__restrict__ px, T *__restrict__ py, T *__restrict__ pz,
+                  T *__restrict__ g_inv, int64_t Np, const uint32_t *__restrict__ np_stay,
+                  T factor_push, int Nx, int Nr, T xmin, T dx_inv, T rmin, T dr_inv,
+                  FieldPtrs fp) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= Np) return;
+  if (!(p < (int64_t)(*np_stay))) return;
+  const int Nx_cell = Nx - 1, Nr_cell = Nr - 1;
+  T xp = x[p], yp = y[p], zp = z[p];
+  T rp = dsqrt<T>(yp * yp + zp * zp);
+  int ix = (int)dfloor<T>((xp - xmin) * dx_inv);
+  int ir = (int)dfloor<T>((rp - rmin) * dr_inv);
+  if (!(ix > 0 && ix < Nx_cell - 1 && ir >= 0 && ir < Nr_cell - 1)) return;
+  int64_t igrid = ix + (int64_t)ir * Nx;
+  T u[3] = {px[p], py[p], pz[p]};
+  T dt_2 = (T)0.5 * factor_push;
+  T rp_inv = (T)1. / rp;
+  T sX1 = (xp - xmin) * dx_inv - (T)ix;
+  T sX0 = (T)1.0 - sX1;
+  T sR1 = (rp - rmin) * dr_inv - (T)ir;
+  T sR0 = (T)1.0 - sR1;
+  // conj(((y + i z)/r)^m) = ((y - i z)/r)^m, m = 1..M
+  T gr[M > 0 ? M + 1 : 2], gi[M > 0 ? M + 1 : 2];
+  gr[1] = yp * rp_inv;
+  gi[1] = -zp * rp_inv;
+#pragma unroll
+  for (int m = 2; m <= M; m++) {
+    gr[m] = gr[m - 1] * gr[1] - gi[m - 1] * gi[1];
+    gi[m] = gr[m - 1] * gi[1] + gi[m - 1] * gr[1];
+  }
+  T C[2][2];
+  C[0][0] = sR0 * sX0;
+  C[0][1] = sR0 * sX1;
+  C[1][0] = sR1 * sX0;
+  C[1][1] = sR1 * sX1;
+  T e_p[3] = {0, 0, 0}, b_p[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        int64_t id = igrid + j + (int64_t)Nx * i;
+        e_p[k] += C[i][j] * ((const T *)fp.e0[k])[id];
+        b_p[k] += C[i][j] * ((const T *)fp.b0[k])[id];
+#pragma unroll
+        for (int m = 1; m <= M; m++) {
+          const T *e = (const T *)fp.ec[k][m - 1];
+          const T *b = (const T *)fp.bc[k][m - 1];
+          T er = 2 * e[2 * id], ei = 2 * e[2 * id + 1];
+          T br = 2 * b[2 * id], bi = 2 * b[2 * id + 1];
+          e_p[k] += C[i][j] * er * gr[m];
+          e_p[k] -= C[i][j] * ei * gi[m];
+          b_p[k] += C[i][j] * br * gr[m];
+          b_p[k] -= C[i][j] * bi * gi[m];
+        }
+      }
+  // Boris rotation, grid_deposit_m1.cl:472-507
+  T um[3], up[3], u0[3], t[3], s[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) um[k] = u[k] + dt_2 * e_p[k];
+  T g_p_inv = (T)1. / dsqrt<T>((T)1. + um[0] * um[0] + um[1] * um[1] + um[2] * um[2]);
+#pragma unroll
+  for (int k = 0; k < 3; k++) t[k] = dt_2 * b_p[k] * g_p_inv;
+  T t2 = (T)2. / ((T)1. + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+#pragma unroll
+  for (int k = 0; k < 3; k++) s[k] = t[k] * t2;
+  u0[0] = um[0] + um[1] * t[2] - um[2] * t[1];
+  u0[1] = um[1] - um[0] * t[2] + um[2] * t[0];
+  u0[2] = um[2] + um[0] * t[1] - um[1] * t[0];
+  up[0] = um[0] + u0[1] * s[2] - u0[2] * s[1];
+  up[1] = um[1] - u0[0] * s[2] + u0[2] * s[0];
+  up[2] = um[2] + u0[0] * s[1] - u0[1] * s[0];
+#pragma unroll
+  for (int k = 0; k < 3; k++) u[k] = up[k] + dt_2 * e_p[k];
+  g_p_inv = (T)1. / dsqrt<T>((T)1. + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  px[p] = u[0];
+  py[p] = u[1];
+  pz[p] = u[2];
+  g_inv[p] = g_p_inv;
+}
+
+template <typename T>
+static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, T *g_inv,
+                       int64_t Np, const uint32_t *np_stay, T factor_push, const chm_grid_t *g,
+                       const chm_planes_t *E, const chm_planes_t *B, void *stream) {
+  if (!g || !E || !B || g->M < 0 || g->M > CHM_MAXM || Np < 0) return CHM_EINVAL;
+  if (Np == 0) return CHM_OK;
+  FieldPtrs fp = {};
+  for (int k = 0; k < 3; k++) {
+    fp.e0[k] = E[k].m0;
+    fp.b0[k] = B[k].m0;
+    for (int m = 0; m < g->M; m++) {
+      fp.ec[k][m] = E[k].mc[m];
+      fp.bc[k][m] = B[k].mc[m];
+    }
+  }
+  hipStream_t st = (hipStream_t)stream;
+  unsigned nb = chm_blocks(Np, 256);
+  switch (g->M) {
+#define CASE(MM)                                                                                   \
+  case MM:                                                                                         \
+    k_gather_push<T, MM><<<nb, 256, 0, st>>>(x, y, z, px, py, pz, g_inv, Np, np_stay, factor_push, \
+                                             g->Nx, g->Nr, (T)g->xmin, (T)g->dx_inv, (T)g->rmin,   \
+                                             (T)g->dr_inv, fp);                                    \
+    break;
+    CASE(0) CASE(1) CASE(2) CASE(3) CASE(4)
+#undef CASE
+  }
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_gather_push_f64(const double *x, const double *y, const double *z, double *px,
+                                   double *py, double *pz, double *g_inv, int64_t Np,
+                                   const uint32_t *np_stay, double factor_push,
+                                   const chm_grid_t *g, const chm_planes_t *E,
+                                   const chm_planes_t *B, void *stream) {
+  return gather_push<double>(x, y, z, px, py, pz, g_inv, Np, np_stay, factor_push, g, E, B, stream);
+}
+extern "C" int chm_gather_push_f32(const float *x, const float *y, const float *z, float *px,
+                                   float *py, float *pz, float *g_inv, int64_t Np,
+                                   const uint32_t *np_stay, float factor_push, const chm_grid_t *g,
+                                   const chm_planes_t *E, const chm_planes_t *B, void *stream) {
+  return gather_push<float>(x, y, z, px, py, pz, g_inv, Np, np_stay, factor_push, g, E, B, stream);
+}

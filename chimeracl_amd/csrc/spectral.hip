@@ -1,0 +1,702 @@
+// Spectral side of the step: Fourier-Bessel transforms (DHT on MFMA + batched
+// row FFT), spectral differential operators, PSATD advance and edge damping.
+//
+// Transforms (transform_field, transformer_methods_cl.py:38-64, 290-455):
+//   forward  F_fb[m] = phase_minus * FFT_x( DHT_m @ F_m[1:] )   (* smooth)
+//   backward F_m[1:] = DHT_inv_m @ Re?( IFFT_x( phase_plus * F_fb[m] ) )
+//   ('Re' for m = 0 only; 'half' mode drops the DHT product).
+// The reference's copy/cast/phase launches are folded into the FFT kernel's
+// load and store; the DHT products for all fields of one mode run as one
+// batched MFMA launch.
+#include "fft.h"
+#include "gemm.h"
+
+#define FFT_NT 256
+#define MAXP 32
+
+template <typename T> struct FftPlane {
+  const T *in;
+  T *out;
+  const T *table; // optional real [nrows][N] post-multiplier
+  int in_cplx, out_cplx;
+};
+template <typename T> struct FftBatch {
+  FftPlane<T> p[MAXP];
+  int nrows;
+  const T *phase_in;  // optional complex [N] pre-multiplier
+  const T *phase_out; // optional complex [N] post-multiplier
+  const T *tw;        // complex [N]: exp(-2 pi i t / N)
+  T scale;
+};
+
+template <int N> struct FftCfg {
+  static constexpr int RPB = (2048 / N) > 0 ? (2048 / N) : 1;
+};
+
+template <typename T, int N, int SIGN>
+__global__ void __launch_bounds__(FFT_NT) k_fft_rows(FftBatch<T> fb) {
+  typedef typename C2<T>::t T2;
+  constexpr int RPB = FftCfg<N>::RPB;
+  __shared__ T2 buf[RPB * N];
+  const FftPlane<T> &pl = fb.p[blockIdx.y];
+  const int row0 = blockIdx.x * RPB;
+  const T2 *phin = (const T2 *)fb.phase_in, *phout = (const T2 *)fb.phase_out;
+  for (int e = threadIdx.x; e < RPB * N; e += FFT_NT) {
+    int row = row0 + e / N, x = e % N;
+    T2 v;
+    v.x = 0;
+    v.y = 0;
+    if (row < fb.nrows) {
+      if (pl.in_cplx) v = ((const T2 *)pl.in)[(int64_t)row * N + x];
+      else v.x = pl.in[(int64_t)row * N + x];
+      if (phin) v = c_mul(v, phin[x]);
+    }
+    buf[e] = v;
+  }
+  __syncthreads();
+  fft_lds<T, N, SIGN, RPB, FFT_NT>(buf, (const T2 *)fb.tw);
+  for (int e = threadIdx.x; e < RPB * N; e += FFT_NT) {
+    int row = row0 + e / N, x = e % N;
+    if (row >= fb.nrows) continue;
+    T2 v = buf[e];
+    v.x *= fb.scale;
+    v.y *= fb.scale;
+    if (phout) v = c_mul(v, phout[x]);
+    if (pl.table) {
+      T t = pl.table[(int64_t)row * N + x];
+      v.x = t * v.x;
+      v.y = t * v.y;
+    }
+    if (pl.out_cplx) ((T2 *)pl.out)[(int64_t)row * N + x] = v;
+    else pl.out[(int64_t)row * N + x] = v.x;
+  }
+}
+
+template <typename T, int SIGN>
+static int launch_fft(const FftBatch<T> &fb, int nplanes, int N, hipStream_t st) {
+  if (nplanes <= 0) return CHM_OK;
+  switch (N) {
+#define CASE(NN)                                                                              \
+  case NN: {                                                                                  \
+    dim3 g((fb.nrows + FftCfg<NN>::RPB - 1) / FftCfg<NN>::RPB, nplanes);                      \
+    k_fft_rows<T, NN, SIGN><<<g, FFT_NT, 0, st>>>(fb);                                        \
+    break;                                                                                    \
+  }
+    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096)
+#undef CASE
+  default:
+    return CHM_EUNSUPPORTED;
+  }
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+static bool fft_size_ok(int N) { return N >= 16 && N <= 4096 && (N & (N - 1)) == 0; }
+
+// ---------------------------------------------------------------- helpers
+static int gemm_kind(const chm_spectral_t *sp, int elem) {
+  if (elem == 8) return GEMM_F64;
+  return sp->dht_kind == CHM_DHT_BF16X3 ? GEMM_BF16X3 : GEMM_F32;
+}
+
+template <typename T> static int elem_of() { return (int)sizeof(T); }
+
+// ---------------------------------------------------------------- transforms
+extern "C" size_t chm_transform_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M, int32_t nflds,
+                                                int32_t elem_bytes) {
+  return chm_align((size_t)nflds * (M + 1) * (Nr - 1) * Nx * 2 * elem_bytes);
+}
+
+template <typename T>
+static int transform(const chm_spectral_t *sp, const chm_planes_t *real, const chm_fb_t *fb,
+                     int32_t nflds, int32_t dir, int32_t mode, int32_t post_smooth, void *ws,
+                     size_t ws_bytes, void *stream) {
+  if (!sp || !real || !fb || nflds <= 0 || sp->M < 0 || sp->M > CHM_MAXM) return CHM_EINVAL;
+  const int Nx = sp->Nx, Nk = sp->Nr - 1, M = sp->M;
+  if (!fft_size_ok(Nx) || Nk < 1) return CHM_EUNSUPPORTED;
+  if (nflds * (M + 1) > MAXP) return CHM_EINVAL;
+  const bool full = mode == 0;
+  size_t plane_elems = (size_t)Nk * Nx * 2;
+  if (full && (chm_transform_workspace_bytes(Nx, sp->Nr, M, nflds, sizeof(T)) > ws_bytes || !ws))
+    return CHM_EWORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  T *scratch = (T *)ws;
+  auto scr = [&](int f, int m) { return scratch + ((size_t)f * (M + 1) + m) * plane_elems; };
+  auto rows1 = [&](int f, int m) -> T * {  // row 1 of the real-space plane
+    if (m == 0) return (T *)real[f].m0 + Nx;
+    return (T *)real[f].mc[m - 1] + 2 * Nx;
+  };
+  const int kind = gemm_kind(sp, sizeof(T));
+  int rc;
+  FftBatch<T> b = {};
+  b.nrows = Nk;
+  b.tw = (const T *)sp->twiddle;
+  if (dir == 0) {
+    if (full) {
+      for (int m = 0; m <= M; m++) {
+        GemmBatch gb = {};
+        gb.M = Nk;
+        gb.K = Nk;
+        gb.N = m == 0 ? Nx : 2 * Nx;
+        gb.lda = Nk;
+        gb.ldb = gb.N;
+        gb.ldc = gb.N;
+        gb.bmode = 0;
+        for (int f = 0; f < nflds; f++) {
+          gb.it[f].A = sp->DHT[m];
+          gb.it[f].A_lo = sp->DHT_lo[m];
+          gb.it[f].X = rows1(f, m);
+          gb.it[f].C = scr(f, m);
+        }
+        if ((rc = chm_gemm_launch(gb, nflds, kind, st))) return rc;
+      }
+    }
+    int np = 0;
+    for (int f = 0; f < nflds; f++)
+      for (int m = 0; m <= M; m++) {
+        FftPlane<T> &p = b.p[np++];
+        p.in = full ? scr(f, m) : rows1(f, m);
+        p.in_cplx = m > 0;
+        p.out = (T *)fb[f].m[m];
+        p.out_cplx = 1;
+        p.table = post_smooth ? (const T *)sp->smooth[m] : nullptr;
+      }
+    b.phase_out = (const T *)sp->phase_minus;
+    b.scale = (T)1;
+    return launch_fft<T, -1>(b, np, Nx, st);
+  }
+  // dir 1
+  int np = 0;
+  for (int f = 0; f < nflds; f++)
+    for (int m = 0; m <= M; m++) {
+      FftPlane<T> &p = b.p[np++];
+      p.in = (const T *)fb[f].m[m];
+      p.in_cplx = 1;
+      p.out = full ? scr(f, m) : rows1(f, m);
+      p.out_cplx = m > 0;
+      p.table = nullptr;
+    }
+  b.phase_in = (const T *)sp->phase_plus;
+  b.scale = (T)1 / (T)Nx;
+  if ((rc = launch_fft<T, 1>(b, np, Nx, st))) return rc;
+  if (full) {
+    for (int m = 0; m <= M; m++) {
+      GemmBatch gb = {};
+      gb.M = Nk;
+      gb.K = Nk;
+      gb.N = m == 0 ? Nx : 2 * Nx;
+      gb.lda = Nk;
+      gb.ldb = gb.N;
+      gb.ldc = gb.N;
+      gb.bmode = 0;
+      for (int f = 0; f < nflds; f++) {
+        gb.it[f].A = sp->DHT_inv[m];
+        gb.it[f].A_lo = sp->DHT_inv_lo[m];
+        gb.it[f].X = scr(f, m);
+        gb.it[f].C = rows1(f, m);
+      }
+      if ((rc = chm_gemm_launch(gb, nflds, kind, st))) return rc;
+    }
+  }
+  return CHM_OK;
+}
+
+extern "C" int chm_transform_f64(const chm_spectral_t *sp, const chm_planes_t *real,
+                                 const chm_fb_t *fb, int32_t nflds, int32_t dir, int32_t mode,
+                                 int32_t post_smooth, void *ws, size_t ws_bytes, void *stream) {
+  return transform<double>(sp, real, fb, nflds, dir, mode, post_smooth, ws, ws_bytes, stream);
+}
+extern "C" int chm_transform_f32(const chm_spectral_t *sp, const chm_planes_t *real,
+                                 const chm_fb_t *fb, int32_t nflds, int32_t dir, int32_t mode,
+                                 int32_t post_smooth, void *ws, size_t ws_bytes, void *stream) {
+  return transform<float>(sp, real, fb, nflds, dir, mode, post_smooth, ws, ws_bytes, stream);
+}
+
+// ---------------------------------------------------------------- table multiply
+struct TabPlanes {
+  void *p[MAXP];
+  const void *t[MAXP];
+};
+
+template <typename T> __global__ void k_mult_table(TabPlanes tp, int64_t n) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  T *z = (T *)tp.p[blockIdx.y];
+  T x = ((const T *)tp.t[blockIdx.y])[e];
+  // mult_elementwise_d2c, generic.cl:47-58
+  z[2 * e] = x * z[2 * e];
+  z[2 * e + 1] = x * z[2 * e + 1];
+}
+
+template <typename T>
+static int mult_table(const chm_spectral_t *sp, const chm_fb_t *flds, int32_t nflds, int32_t which,
+                      void *stream) {
+  if (!sp || !flds || nflds <= 0 || nflds * (sp->M + 1) > MAXP) return CHM_EINVAL;
+  TabPlanes tp = {};
+  int np = 0;
+  for (int m = 0; m <= sp->M; m++)
+    for (int f = 0; f < nflds; f++) {
+      tp.p[np] = flds[f].m[m];
+      tp.t[np++] = which == 0 ? sp->smooth[m] : sp->c3[m];
+    }
+  int64_t n = (int64_t)(sp->Nr - 1) * sp->Nx;
+  dim3 g(chm_blocks(n, 256), np);
+  k_mult_table<T><<<g, 256, 0, (hipStream_t)stream>>>(tp, n);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_mult_table_f64(const chm_spectral_t *sp, const chm_fb_t *flds, int32_t nflds,
+                                  int32_t which, void *stream) {
+  return mult_table<double>(sp, flds, nflds, which, stream);
+}
+extern "C" int chm_mult_table_f32(const chm_spectral_t *sp, const chm_fb_t *flds, int32_t nflds,
+                                  int32_t which, void *stream) {
+  return mult_table<float>(sp, flds, nflds, which, stream);
+}
+
+// ---------------------------------------------------------------- grad / div / rot
+// Scratch: up to 4 complex [Nk][Nx] planes per mode.
+extern "C" size_t chm_spectral_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M, int32_t elem_bytes) {
+  return chm_align((size_t)4 * (M + 1) * (Nr - 1) * Nx * 2 * elem_bytes);
+}
+
+struct OpPlanes {
+  // per mode m: inputs / outputs / products
+  const void *in[3][CHM_MAXM + 1];   // op inputs at mode m (comp 0..2)
+  void *out[3][CHM_MAXM + 1];        // outputs
+  void *prev[3][CHM_MAXM + 1];       // grad: old output copy target (nullable)
+  const void *P[4][CHM_MAXM + 1];    // products (nullable)
+  const void *tab[CHM_MAXM + 1];     // rot: Poisson table (nullable)
+};
+
+// op 0 grad: in[0] = scl; out x,y,z; P0 = D- S_{m-1}, P1 = D+ S_{m+1}
+// op 1 div:  in = V x,y,z; out[0]; P0 = D-(-iVz - Vy)_{m-1}, P1 = D+(-iVz + Vy)_{m+1}
+// op 2 rot:  in = G x,y,z; out x,y,z; P0 = D-(-Gz + iGy), P1 = D- Gx, P2 = D+(Gz + iGy), P3 = D+ Gx
+template <typename T, int OP>
+__global__ void k_assemble(OpPlanes op, const T *__restrict__ kx, int Nx, int64_t n) {
+  typedef typename C2<T>::t T2;
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int m = blockIdx.y;
+  if (e >= n) return;
+  int ix = (int)(e % Nx);
+  T k = kx[ix];
+  auto ld = [&](const void *p) -> T2 {
+    T2 z;
+    z.x = 0;
+    z.y = 0;
+    if (p) z = ((const T2 *)p)[e];
+    return z;
+  };
+  auto st = [&](void *p, T2 v) { ((T2 *)p)[e] = v; };
+  // ab_dot_x(a, kx, x): z = kx * (a*x) (generic.cl:82-99)
+  auto kdot = [&](T ar, T ai, T2 x) {
+    T2 z;
+    z.x = k * (ar * x.x - ai * x.y);
+    z.y = k * (ar * x.y + ai * x.x);
+    return z;
+  };
+  if (OP == 0) {
+    if (op.prev[0][m])
+      for (int c = 0; c < 3; c++) st(op.prev[c][m], ld(op.out[c][m]));
+    T2 s = ld(op.in[0][m]);
+    T2 P = ld(op.P[0][m]), Q = ld(op.P[1][m]);
+    T2 x = kdot(0, 1, s);
+    T2 y, z;
+    y.x = -P.x + Q.x;
+    y.y = -P.y + Q.y;
+    // -i*P - i*Q
+    z.x = P.y + Q.y;
+    z.y = -P.x - Q.x;
+    st(op.out[0][m], x);
+    st(op.out[1][m], y);
+    st(op.out[2][m], z);
+  } else if (OP == 1) {
+    T2 vx = ld(op.in[0][m]);
+    T2 P = ld(op.P[0][m]), Q = ld(op.P[1][m]);
+    T2 o = kdot(0, 1, vx);
+    o.x = o.x + P.x + Q.x;
+    o.y = o.y + P.y + Q.y;
+    st(op.out[0][m], o);
+  } else {
+    T2 gy = ld(op.in[1][m]), gz = ld(op.in[2][m]);
+    T2 P1 = ld(op.P[0][m]), P2 = ld(op.P[1][m]), Q1 = ld(op.P[2][m]), Q2 = ld(op.P[3][m]);
+    T2 bx, by, bz;
+    bx.x = P1.x + Q1.x;
+    bx.y = P1.y + Q1.y;
+    by = kdot(0, -1, gz);
+    // -i*(P2 + Q2)
+    by.x += P2.y + Q2.y;
+    by.y += -P2.x - Q2.x;
+    bz = kdot(0, 1, gy);
+    bz.x += P2.x - Q2.x;
+    bz.y += P2.y - Q2.y;
+    if (op.tab[m]) {
+      T t = ((const T *)op.tab[m])[e];
+      bx.x *= t; bx.y *= t;
+      by.x *= t; by.y *= t;
+      bz.x *= t; bz.y *= t;
+    }
+    st(op.out[0][m], bx);
+    st(op.out[1][m], by);
+    st(op.out[2][m], bz);
+  }
+}
+
+template <typename T>
+static int spectral_op(int OP, const chm_spectral_t *sp, const chm_fb_t *in, const chm_fb_t *out,
+                       const chm_fb_t *prev, int32_t poisson, void *ws, size_t ws_bytes,
+                       void *stream) {
+  if (!sp || !in || !out || sp->M < 0 || sp->M > CHM_MAXM) return CHM_EINVAL;
+  const int Nx = sp->Nx, Nk = sp->Nr - 1, M = sp->M;
+  if (chm_spectral_workspace_bytes(Nx, sp->Nr, M, sizeof(T)) > ws_bytes || !ws) return CHM_EWORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t pe = (size_t)Nk * Nx * 2;
+  T *scr = (T *)ws;
+  OpPlanes op = {};
+  GemmBatch gb = {};
+  gb.M = Nk;
+  gb.K = Nk;
+  gb.N = 2 * Nx;
+  gb.Nx = Nx;
+  gb.lda = Nk;
+  gb.ldc = 2 * Nx;
+  gb.bmode = 1;
+  int ni = 0, ns = 0;
+  const int kind = gemm_kind(sp, sizeof(T));
+  int rc;
+  auto item = [&](const void *A, const void *Alo, const void *X, double2 a, int rx, const void *Y,
+                  double2 b, int ry) -> const void * {
+    if (ni == GEMM_MAXB) {
+      if ((rc = chm_gemm_launch(gb, ni, kind, st))) return nullptr;
+      ni = 0;
+    }
+    GemmItem &it = gb.it[ni++];
+    it.A = A;
+    it.A_lo = Alo;
+    it.X = X;
+    it.Y = Y;
+    it.alpha = a;
+    it.beta = b;
+    it.remapX = rx;
+    it.remapY = ry;
+    it.C = scr + (size_t)(ns++) * pe;
+    return it.C;
+  };
+  const double2 one = {1, 0}, mone = {-1, 0}, I = {0, 1}, mI = {0, -1}, zero = {0, 0};
+  const int ncomp_in = (OP == 0) ? 1 : 3;
+  for (int m = 0; m <= M; m++)
+    for (int c = 0; c < ncomp_in; c++) op.in[c][m] = in[c].m[m];
+  for (int m = 0; m <= M; m++)
+    for (int c = 0; c < (OP == 1 ? 1 : 3); c++) {
+      op.out[c][m] = out[c].m[m];
+      if (prev) op.prev[c][m] = prev[c].m[m];
+    }
+  rc = CHM_OK;
+  for (int m = 0; m <= M; m++) {
+    const bool lower = m > 0 || M > 0;  // an m-1 neighbour exists (m = 0 uses F_{-1})
+    const int mm = m > 0 ? m - 1 : 1;   // source mode of the m-1 term
+    const int rm = m > 0 ? 0 : 1;       // read it through the F_{-1} map
+    if (OP == 0) {
+      if (lower) op.P[0][m] = item(sp->dDm[m], sp->dDm_lo[m], in[0].m[mm], one, rm, nullptr, zero, 0);
+      if (m < M) op.P[1][m] = item(sp->dDp[m], sp->dDp_lo[m], in[0].m[m + 1], one, 0, nullptr, zero, 0);
+    } else if (OP == 1) {
+      if (lower) op.P[0][m] = item(sp->dDm[m], sp->dDm_lo[m], in[2].m[mm], mI, rm, in[1].m[mm], mone, rm);
+      if (m < M) op.P[1][m] = item(sp->dDp[m], sp->dDp_lo[m], in[2].m[m + 1], mI, 0, in[1].m[m + 1], one, 0);
+    } else {
+      if (lower) {
+        op.P[0][m] = item(sp->dDm[m], sp->dDm_lo[m], in[2].m[mm], mone, rm, in[1].m[mm], I, rm);
+        op.P[1][m] = item(sp->dDm[m], sp->dDm_lo[m], in[0].m[mm], one, rm, nullptr, zero, 0);
+      }
+      if (m < M) {
+        op.P[2][m] = item(sp->dDp[m], sp->dDp_lo[m], in[2].m[m + 1], one, 0, in[1].m[m + 1], I, 0);
+        op.P[3][m] = item(sp->dDp[m], sp->dDp_lo[m], in[0].m[m + 1], one, 0, nullptr, zero, 0);
+      }
+      if (poisson) op.tab[m] = sp->c3[m];
+    }
+    if (rc) return rc;
+  }
+  if ((rc = chm_gemm_launch(gb, ni, kind, st))) return rc;
+  int64_t n = (int64_t)Nk * Nx;
+  dim3 g(chm_blocks(n, 256), M + 1);
+  if (OP == 0) k_assemble<T, 0><<<g, 256, 0, st>>>(op, (const T *)sp->kx, Nx, n);
+  else if (OP == 1) k_assemble<T, 1><<<g, 256, 0, st>>>(op, (const T *)sp->kx, Nx, n);
+  else k_assemble<T, 2><<<g, 256, 0, st>>>(op, (const T *)sp->kx, Nx, n);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_field_grad_f64(const chm_spectral_t *sp, const chm_fb_t *scl, const chm_fb_t *out,
+                                  const chm_fb_t *prev, void *ws, size_t ws_bytes, void *stream) {
+  return spectral_op<double>(0, sp, scl, out, prev, 0, ws, ws_bytes, stream);
+}
+extern "C" int chm_field_grad_f32(const chm_spectral_t *sp, const chm_fb_t *scl, const chm_fb_t *out,
+                                  const chm_fb_t *prev, void *ws, size_t ws_bytes, void *stream) {
+  return spectral_op<float>(0, sp, scl, out, prev, 0, ws, ws_bytes, stream);
+}
+extern "C" int chm_field_div_f64(const chm_spectral_t *sp, const chm_fb_t *vec, const chm_fb_t *out,
+                                 void *ws, size_t ws_bytes, void *stream) {
+  return spectral_op<double>(1, sp, vec, out, nullptr, 0, ws, ws_bytes, stream);
+}
+extern "C" int chm_field_div_f32(const chm_spectral_t *sp, const chm_fb_t *vec, const chm_fb_t *out,
+                                 void *ws, size_t ws_bytes, void *stream) {
+  return spectral_op<float>(1, sp, vec, out, nullptr, 0, ws, ws_bytes, stream);
+}
+extern "C" int chm_field_rot_f64(const chm_spectral_t *sp, const chm_fb_t *in, const chm_fb_t *out,
+                                 int32_t poisson, void *ws, size_t ws_bytes, void *stream) {
+  return spectral_op<double>(2, sp, in, out, nullptr, poisson, ws, ws_bytes, stream);
+}
+extern "C" int chm_field_rot_f32(const chm_spectral_t *sp, const chm_fb_t *in, const chm_fb_t *out,
+                                 int32_t poisson, void *ws, size_t ws_bytes, void *stream) {
+  return spectral_op<float>(2, sp, in, out, nullptr, poisson, ws, ws_bytes, stream);
+}
+
+// ---------------------------------------------------------------- PSATD
+struct SolvPlanes {
+  void *e[3][CHM_MAXM + 1], *g[3][CHM_MAXM + 1];
+  const void *j[3][CHM_MAXM + 1], *n0[3][CHM_MAXM + 1], *n1[3][CHM_MAXM + 1];
+  const void *c1[CHM_MAXM + 1], *c2[CHM_MAXM + 1], *c3[CHM_MAXM + 1];
+};
+
+// advance_e_g_m, solver_ms_pic.cl:187-213 (same expression order)
+template <typename T>
+__device__ __forceinline__ void psatd(T c1, T c2, T c3, T dt_inv, T e0, T g0, T j0, T n0, T n1,
+                                      T &e1, T &g1) {
+  const T pi2 = (T)(2 * 3.14159265358979323846);
+  j0 *= pi2;
+  n0 *= pi2;
+  n1 *= pi2;
+  e1 = c1 * e0 + c2 * c3 * (g0 - j0) + c3 * (c1 * n0 - n1 - (n0 - n1) * dt_inv * c2 * c3);
+  g1 = -c2 * e0 + c1 * (g0 - j0) + j0 + c3 * (dt_inv * ((T)1. - c1) * (n0 - n1) - c2 * n0);
+}
+
+template <typename T> __global__ void k_advance(SolvPlanes sv, T dt_inv, int64_t n) {
+  typedef typename C2<T>::t T2;
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int m = blockIdx.y;
+  if (e >= n) return;
+  T c1 = ((const T *)sv.c1[m])[e], c2 = ((const T *)sv.c2[m])[e], c3 = ((const T *)sv.c3[m])[e];
+  for (int k = 0; k < 3; k++) {
+    T2 e0 = ((T2 *)sv.e[k][m])[e], g0 = ((T2 *)sv.g[k][m])[e];
+    T2 j0 = ((const T2 *)sv.j[k][m])[e], n0 = ((const T2 *)sv.n0[k][m])[e], n1 = ((const T2 *)sv.n1[k][m])[e];
+    T2 e1, g1;
+    psatd<T>(c1, c2, c3, dt_inv, e0.x, g0.x, j0.x, n0.x, n1.x, e1.x, g1.x);
+    psatd<T>(c1, c2, c3, dt_inv, e0.y, g0.y, j0.y, n0.y, n1.y, e1.y, g1.y);
+    ((T2 *)sv.e[k][m])[e] = e1;
+    ((T2 *)sv.g[k][m])[e] = g1;
+  }
+}
+
+static void fill_solv(SolvPlanes &sv, const chm_spectral_t *sp, const chm_fb_t *E, const chm_fb_t *G,
+                      const chm_fb_t *J, const chm_fb_t *dN0, const chm_fb_t *dN1) {
+  for (int m = 0; m <= sp->M; m++) {
+    for (int k = 0; k < 3; k++) {
+      sv.e[k][m] = E[k].m[m];
+      sv.g[k][m] = G[k].m[m];
+      if (J) {
+        sv.j[k][m] = J[k].m[m];
+        sv.n0[k][m] = dN0[k].m[m];
+        sv.n1[k][m] = dN1[k].m[m];
+      }
+    }
+    sv.c1[m] = sp->c1[m];
+    sv.c2[m] = sp->c2[m];
+    sv.c3[m] = sp->c3[m];
+  }
+}
+
+template <typename T>
+static int advance(const chm_spectral_t *sp, const chm_fb_t *E, const chm_fb_t *G, const chm_fb_t *J,
+                   const chm_fb_t *dN0, const chm_fb_t *dN1, void *stream) {
+  if (!sp || !E || !G || !J || !dN0 || !dN1 || sp->M < 0 || sp->M > CHM_MAXM) return CHM_EINVAL;
+  SolvPlanes sv = {};
+  fill_solv(sv, sp, E, G, J, dN0, dN1);
+  int64_t n = (int64_t)(sp->Nr - 1) * sp->Nx;
+  dim3 g(chm_blocks(n, 256), sp->M + 1);
+  k_advance<T><<<g, 256, 0, (hipStream_t)stream>>>(sv, (T)sp->dt_inv, n);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_advance_fields_f64(const chm_spectral_t *sp, const chm_fb_t *E, const chm_fb_t *G,
+                                      const chm_fb_t *J, const chm_fb_t *dN0, const chm_fb_t *dN1,
+                                      void *stream) {
+  return advance<double>(sp, E, G, J, dN0, dN1, stream);
+}
+extern "C" int chm_advance_fields_f32(const chm_spectral_t *sp, const chm_fb_t *E, const chm_fb_t *G,
+                                      const chm_fb_t *J, const chm_fb_t *dN0, const chm_fb_t *dN1,
+                                      void *stream) {
+  return advance<float>(sp, E, G, J, dN0, dN1, stream);
+}
+
+// ---------------------------------------------------------------- damping
+// One block per (row kr, mode m, component c): optional PSATD of the row,
+// then for E and G: *phase_plus -> IFFT -> (m=0: real part) -> *profile(x)
+// -> FFT -> *phase_minus.  (Solver.damp_fields, solver.py:32-35.)
+template <typename T, int N, bool ADV>
+__global__ void __launch_bounds__(FFT_NT) k_damp(SolvPlanes sv, const T *__restrict__ damp, int nd,
+                                                 const T *__restrict__ php, const T *__restrict__ phm,
+                                                 const T *__restrict__ tw, T dt_inv, int Nk) {
+  typedef typename C2<T>::t T2;
+  constexpr int PER = (N + FFT_NT - 1) / FFT_NT;
+  __shared__ T2 buf[N];
+  const int kr = blockIdx.x, m = blockIdx.y, c = blockIdx.z;
+  const int64_t rowoff = (int64_t)kr * N;
+  T2 ev[PER], gv[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int x = threadIdx.x + q * FFT_NT;
+    if (x < N) {
+      T2 e0 = ((T2 *)sv.e[c][m])[rowoff + x], g0 = ((T2 *)sv.g[c][m])[rowoff + x];
+      if (ADV) {
+        int64_t id = rowoff + x;
+        T c1 = ((const T *)sv.c1[m])[id], c2 = ((const T *)sv.c2[m])[id], c3 = ((const T *)sv.c3[m])[id];
+        T2 j0 = ((const T2 *)sv.j[c][m])[id], n0 = ((const T2 *)sv.n0[c][m])[id], n1 = ((const T2 *)sv.n1[c][m])[id];
+        T2 e1, g1;
+        psatd<T>(c1, c2, c3, dt_inv, e0.x, g0.x, j0.x, n0.x, n1.x, e1.x, g1.x);
+        psatd<T>(c1, c2, c3, dt_inv, e0.y, g0.y, j0.y, n0.y, n1.y, e1.y, g1.y);
+        e0 = e1;
+        g0 = g1;
+      }
+      ev[q] = e0;
+      gv[q] = g0;
+    }
+  }
+  const T inv = (T)1 / (T)N;
+  for (int h = 0; h < 2; h++) {
+    T2 *vv = h == 0 ? ev : gv;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      int x = threadIdx.x + q * FFT_NT;
+      if (x < N) buf[x] = c_mul(vv[q], ((const T2 *)php)[x]);
+    }
+    __syncthreads();
+    fft_lds<T, N, 1, 1, FFT_NT>(buf, (const T2 *)tw);
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      int x = threadIdx.x + q * FFT_NT;
+      if (x < N) {
+        T2 v = buf[x];
+        v.x *= inv;
+        v.y *= inv;
+        if (m == 0) v.y = 0;
+        if (x < nd) { v.x *= damp[x]; v.y *= damp[x]; }
+        if (x > N - nd) { v.x *= damp[N - x]; v.y *= damp[N - x]; }
+        buf[x] = v;
+      }
+    }
+    __syncthreads();
+    fft_lds<T, N, -1, 1, FFT_NT>(buf, (const T2 *)tw);
+    void *dst = h == 0 ? sv.e[c][m] : sv.g[c][m];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      int x = threadIdx.x + q * FFT_NT;
+      if (x < N) ((T2 *)dst)[rowoff + x] = c_mul(buf[x], ((const T2 *)phm)[x]);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+static int damp_fields(const chm_spectral_t *sp, const chm_fb_t *E, const chm_fb_t *G, int32_t adv,
+                       const chm_fb_t *J, const chm_fb_t *dN0, const chm_fb_t *dN1, void *stream) {
+  if (!sp || !E || !G || sp->M < 0 || sp->M > CHM_MAXM) return CHM_EINVAL;
+  if (adv && (!J || !dN0 || !dN1)) return CHM_EINVAL;
+  if (sp->ndamp <= 0 || !sp->damp) {
+    if (adv) return advance<T>(sp, E, G, J, dN0, dN1, stream);
+    return CHM_OK;
+  }
+  const int Nx = sp->Nx;
+  if (!fft_size_ok(Nx)) return CHM_EUNSUPPORTED;
+  SolvPlanes sv = {};
+  fill_solv(sv, sp, E, G, adv ? J : nullptr, dN0, dN1);
+  dim3 g(sp->Nr - 1, sp->M + 1, 3);
+  hipStream_t st = (hipStream_t)stream;
+  const T *dp = (const T *)sp->damp, *php = (const T *)sp->phase_plus, *phm = (const T *)sp->phase_minus,
+          *tw = (const T *)sp->twiddle;
+  T dti = (T)sp->dt_inv;
+  switch (Nx) {
+#define CASE(NN)                                                                                       \
+  case NN:                                                                                             \
+    if (adv) k_damp<T, NN, true><<<g, FFT_NT, 0, st>>>(sv, dp, sp->ndamp, php, phm, tw, dti, sp->Nr - 1); \
+    else k_damp<T, NN, false><<<g, FFT_NT, 0, st>>>(sv, dp, sp->ndamp, php, phm, tw, dti, sp->Nr - 1); \
+    break;
+    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096)
+#undef CASE
+  }
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_damp_fields_f64(const chm_spectral_t *sp, const chm_fb_t *E, const chm_fb_t *G,
+                                   int32_t advance_first, const chm_fb_t *J, const chm_fb_t *dN0,
+                                   const chm_fb_t *dN1, void *stream) {
+  return damp_fields<double>(sp, E, G, advance_first, J, dN0, dN1, stream);
+}
+extern "C" int chm_damp_fields_f32(const chm_spectral_t *sp, const chm_fb_t *E, const chm_fb_t *G,
+                                   int32_t advance_first, const chm_fb_t *J, const chm_fb_t *dN0,
+                                   const chm_fb_t *dN1, void *stream) {
+  return damp_fields<float>(sp, E, G, advance_first, J, dN0, dN1, stream);
+}
+
+// ---------------------------------------------------------------- laser
+// laser.py:17-19
+__global__ void k_laser_pulse(double *Ez, int Nx, int Nr, double xmin, double dx, double rmin,
+                              double dr, double k0, double a0, double x0, double Lx, double R) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)(Nr - 1) * Nx) return;
+  int ir = 1 + (int)(e / Nx), ix = (int)(e % Nx);
+  double x = xmin + dx * (double)ix;
+  double r = rmin + dr * (double)ir;
+  double xs = x - x0;
+  double v = a0 * sin(k0 * xs) * exp(-xs * xs / (Lx * Lx) - r * r / (R * R));
+  if (!(fabs(r) < 3.5 * R) || !(fabs(xs) < 3.5 * Lx)) v = 0.0 * v;
+  Ez[(int64_t)ir * Nx + ix] = v;
+}
+
+extern "C" int chm_laser_pulse_f64(double *Ez_m0, int32_t Nx, int32_t Nr, double xmin, double dx,
+                                   double rmin, double dr, double k0, double a0, double x0,
+                                   double Lx, double R, void *stream) {
+  if (!Ez_m0 || Nx < 1 || Nr < 2) return CHM_EINVAL;
+  int64_t n = (int64_t)(Nr - 1) * Nx;
+  k_laser_pulse<<<chm_blocks(n, 256), 256, 0, (hipStream_t)stream>>>(Ez_m0, Nx, Nr, xmin, dx, rmin,
+                                                                      dr, k0, a0, x0, Lx, R);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+// laser.py:24-34 on mode 0
+__global__ void k_laser_spectral(const double *kx, const double *w, double2 *E, double2 *G, double X,
+                                 int Nx, int64_t n) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int ix = (int)(e % Nx);
+  double k = kx[ix], ww = w[e];
+  double sg = (k + (k == 0 ? 1.0 : 0.0)) > 0 ? 1.0 : -1.0;
+  double2 EE = E[e];
+  // DT = -i w sign(kx); GG = DT*EE
+  double2 GG;
+  GG.x = ww * sg * EE.y;
+  GG.y = -ww * sg * EE.x;
+  double cw = cos(ww * X), sw = sin(ww * X);
+  double2 Et, Gt;
+  Et.x = cw * EE.x + sw / ww * GG.x;
+  Et.y = cw * EE.y + sw / ww * GG.y;
+  Gt.x = -ww * sw * EE.x + cw * GG.x;
+  Gt.y = -ww * sw * EE.y + cw * GG.y;
+  double c = cos(k * X), s = sin(k * X);
+  E[e].x = Et.x * c - Et.y * s;
+  E[e].y = Et.x * s + Et.y * c;
+  G[e].x = Gt.x * c - Gt.y * s;
+  G[e].y = Gt.x * s + Gt.y * c;
+}
+
+extern "C" int chm_laser_spectral_f64(const chm_spectral_t *sp, const double *w_m0, void *Ez_fb_m0,
+                                      void *Gz_fb_m0, double X_focus, void *stream) {
+  if (!sp || !w_m0 || !Ez_fb_m0 || !Gz_fb_m0) return CHM_EINVAL;
+  int64_t n = (int64_t)(sp->Nr - 1) * sp->Nx;
+  k_laser_spectral<<<chm_blocks(n, 256), 256, 0, (hipStream_t)stream>>>(
+      (const double *)sp->kx, w_m0, (double2 *)Ez_fb_m0, (double2 *)Gz_fb_m0, X_focus, sp->Nx, n);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}

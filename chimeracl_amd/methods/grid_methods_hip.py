@@ -1,0 +1,104 @@
+"""Grid methods (replaces chimeraCL/methods/grid_methods_cl.py): deposits,
+axis post-processing, ghost-row warp and the gather + push."""
+from __future__ import annotations
+
+import torch
+
+from .. import _lib
+from .generic_methods_hip import GenericMethodsHIP, ptr
+from .particles_methods_hip import _grid_struct
+
+
+class GridMethodsHIP(GenericMethodsHIP):
+    def init_grid_methods(self):
+        self.set_global_working_group_size()
+        if "vec_comps" not in self.Args:
+            self.Args["vec_comps"] = ["x", "y", "z"]
+        self._pcache = {}
+
+    # ---------------------------------------------------------------- structs
+    def _planes(self, name):
+        """chm_planes_t of the real-space planes '<name>_m<m>'."""
+        st = self._pcache.get(name)
+        if st is None:
+            M = self.Args["M"]
+            st = _lib.Planes()
+            st.m0 = ptr(self.DataDev[name + "_m0"])
+            for m in range(1, M + 1):
+                st.mc[m - 1] = ptr(self.DataDev[f"{name}_m{m}"])
+            self._pcache[name] = st
+        return st
+
+    def _planes_arr(self, names):
+        key = ("arr",) + tuple(names)
+        arr = self._pcache.get(key)
+        if arr is None:
+            arr = (_lib.Planes * len(names))(*[self._planes(n) for n in names])
+            self._pcache[key] = arr
+        return arr
+
+    def _grid(self):
+        g = self._pcache.get("grid")
+        if g is None:
+            g = self._pcache["grid"] = _grid_struct(self)
+        return g
+
+    def _dep_ws(self):
+        ws = self._pcache.get("dep_ws")
+        if ws is None:
+            a = self.Args
+            n = _lib.lib.chm_deposit_workspace_bytes(a["Nx"], a["Nr"], a["M"], 1)
+            ws = self._pcache["dep_ws"] = torch.empty(n, dtype=torch.uint8, device=self.comm.device)
+        return ws
+
+    # ---------------------------------------------------------------- deposits
+    def depose_scalar(self, parts, src_scalar, dest_fld, charge):
+        """grid_methods_cl.py:44-69 (adds into '<dest_fld>_m*')."""
+        if parts.Args["Np"] <= 0:
+            return
+        d, ws = parts.DataDev, self._dep_ws()
+        self.call("chm_deposit_scalar", ptr(d["sort_indx"]), ptr(d["cell_offset"]), ptr(d["x"]),
+                  ptr(d["y"]), ptr(d["z"]), ptr(d[src_scalar]), int(charge), self._grid(),
+                  self._planes_arr([dest_fld]), ptr(ws), ws.numel())
+
+    def depose_vector(self, parts, vec, factors, vec_fld, charge):
+        """grid_methods_cl.py:71-95 (adds w*g_inv*charge*u into '<vec_fld><c>_m*')."""
+        if parts.Args["Np"] <= 0:
+            return
+        d, ws = parts.DataDev, self._dep_ws()
+        comps = self.Args["vec_comps"]
+        self.call("chm_deposit_vector", ptr(d["sort_indx"]), ptr(d["cell_offset"]), ptr(d["x"]),
+                  ptr(d["y"]), ptr(d["z"]), *(ptr(d[v]) for v in vec), ptr(d[factors[0]]),
+                  ptr(d[factors[1]]), int(charge), self._grid(),
+                  self._planes_arr([vec_fld + c for c in comps]), ptr(ws), ws.numel())
+
+    def _postproc(self, names):
+        a = self.Args
+        self.call("chm_postproc_deposit", self._planes_arr(names), len(names),
+                  ptr(self.DataDev["dV_inv"]), a["Nx"], a["Nr"], a["M"])
+
+    def postproc_depose_scalar(self, fld):
+        """grid_methods_cl.py:98-124: treat_axis then divide by dV."""
+        self._postproc([fld])
+
+    def postproc_depose_vector(self, vec_fld):
+        """grid_methods_cl.py:126-151."""
+        self._postproc([vec_fld + c for c in self.Args["vec_comps"]])
+
+    def preproc_project_vec(self, vec_fld):
+        """grid_methods_cl.py:153-166: ghost row of E or B."""
+        a = self.Args
+        names = [vec_fld + c for c in self.Args["vec_comps"]]
+        self.call("chm_warp_axis", self._planes_arr(names), len(names), a["Nx"], a["Nr"], a["M"])
+
+    def _gather_and_push(self, parts, flds):
+        """grid_methods_cl.py:168-192."""
+        if parts.Args["Np"] == 0:
+            return
+        d = parts.DataDev
+        comps = self.Args["vec_comps"]
+        E = self._planes_arr([flds[0] + c for c in comps])
+        B = self._planes_arr([flds[1] + c for c in comps])
+        self.call("chm_gather_push", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]),
+                  ptr(d["py"]), ptr(d["pz"]), ptr(d["g_inv"]), parts.Args["Np"],
+                  ptr(d["Np_stay"]), parts.Args["FactorPush"], self._grid(), E, B)

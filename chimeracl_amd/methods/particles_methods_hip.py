@@ -1,0 +1,150 @@
+"""Particle methods (replaces chimeraCL/methods/particles_methods_cl.py).
+
+push_coords, index_sort, dens_profile and the particle bookkeeping call the
+HIP kernels of libchimera_hip.so.  Random plasma/beam generation on the
+device (make_new_domain / make_new_beam, Threefry) is not part of this
+build: particles come in through add_particles(source=...), e.g. from
+chimeracl_amd.config.load_plasma.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .generic_methods_hip import GenericMethodsHIP, LazyArgs, ptr
+
+
+def _grid_struct(grid):
+    a = grid.Args
+    return _lib.Grid(Nx=a["Nx"], Nr=a["Nr"], M=a["M"], xmin=a["Xmin"], dx_inv=a["dx_inv"],
+                     rmin=a["Rmin"], dr_inv=a["dr_inv"])
+
+
+class ParticleMethodsHIP(GenericMethodsHIP):
+    def init_particle_methods(self):
+        self.set_global_working_group_size()
+        self._sort_ws = None
+        self._np_stay_dev = None
+
+    def _arg_names(self):
+        if "Immobile" in self.Args:
+            return ["x", "y", "z", "w"]
+        return ["x", "y", "z", "px", "py", "pz", "w", "g_inv"]
+
+    # ---------------------------------------------------------------- data
+    def add_new_particles(self, source=None):
+        """particles_methods_cl.py:40-61 (appends source.DataDev['<c>_new'])."""
+        src = self.DataDev if source is None else source.DataDev
+        real = self.comm.real
+        for arg in self._arg_names():
+            new = src[arg + "_new"]
+            new = (torch.from_numpy(np.ascontiguousarray(new)) if isinstance(new, np.ndarray) else new)
+            new = new.to(self.comm.device, real)
+            old = self.DataDev.get(arg)
+            self.DataDev[arg] = new.contiguous().clone() if (old is None or old.numel() == 0) \
+                else torch.cat([old, new])
+        self.reset_num_parts()
+        self.flag_sorted = False
+
+    def make_new_domain(self, parts_in, density_profiles=None):
+        raise NotImplementedError(
+            "device-side random plasma loading is not part of this build; load particles with "
+            "chimeracl_amd.config.load_plasma and add_particles(source=...)")
+
+    def make_new_beam(self, parts_in):
+        raise NotImplementedError(
+            "device-side random beam loading is not part of this build; pass arrays through "
+            "add_particles(source=...)")
+
+    def reset_num_parts(self, Np=None):
+        if Np is None:
+            Np = int(self.DataDev["x"].numel())
+        self.Args["Np"] = Np
+        self.Args["Np_stay"] = Np
+        self._sort_ws = None
+
+    # ---------------------------------------------------------------- profile
+    def dens_profile(self, x_prf, f_prf, xmin=None, xmax=None, coord="x", weight="w"):
+        """particles_methods_cl.py:179-204 + profile_by_interpolant
+        (particles_generic.cl:6-30); constant extension outside the knots."""
+        xx = np.ascontiguousarray(x_prf, dtype=np.float64)
+        ff = np.ascontiguousarray(f_prf, dtype=np.float64)
+        dxm1 = 1.0 / (xx[1:] - xx[:-1])
+        dev = self.comm.device
+        self._prof = [torch.from_numpy(a).to(dev) for a in (xx, ff, dxm1)]
+        x, w = self.DataDev[coord], self.DataDev[weight]
+        self.call("chm_dens_profile", ptr(x), ptr(w), x.numel(), *(ptr(t) for t in self._prof),
+                  xx.size)
+
+    # ---------------------------------------------------------------- push
+    def push_coords(self, mode="half"):
+        """particles_methods_cl.py:206-223."""
+        if self.Args["Np"] == 0 or "Immobile" in self.Args:
+            return
+        dt = self.Args["dt_2"] if mode == "half" else self.Args["dt"]
+        d = self.DataDev
+        self.call("chm_push_x", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]),
+                  ptr(d["py"]), ptr(d["pz"]), ptr(d["g_inv"]), dt, self.Args["Np"])
+        self.flag_sorted = False
+
+    # ---------------------------------------------------------------- sort
+    def _ensure_sort_buffers(self, grid):
+        Np, nbins = self.Args["Np"], grid.Args["Nxm1Nrm1"] + 1
+        key = (Np, nbins)
+        if self._sort_ws is not None and self._sort_ws[0] == key:
+            return
+        dev = self.comm.device
+        nbytes = _lib.lib.chm_sort_workspace_bytes(Np, nbins)
+        ws = torch.zeros(nbytes, dtype=torch.uint8, device=dev)
+        self.DataDev["indx_in_cell"] = torch.empty(Np, dtype=torch.int32, device=dev)
+        self.DataDev["sort_indx"] = torch.empty(Np, dtype=torch.int32, device=dev)
+        self.DataDev["cell_offset"] = torch.empty(nbins + 1, dtype=torch.int32, device=dev)
+        self.DataDev["Np_stay"] = torch.zeros(1, dtype=torch.int32, device=dev)
+        foff = _lib.lib.chm_sort_flags_offset(Np, nbins)
+        self.DataDev["sum_in_cell"] = ws[: nbins * 4].view(torch.int32)
+        self._sort_flags = ws[foff: foff + 4].view(torch.int32)
+        self._sort_ws = (key, ws, _grid_struct(grid))
+
+    def index_sort(self, grid):
+        """particles_methods_cl.py:225-261: cell keys, histogram, offsets and
+        the stable in-cell order; Np_stay stays on the device and is read
+        lazily from Args."""
+        self._ensure_sort_buffers(grid)
+        _, ws, gs = self._sort_ws
+        d = self.DataDev
+        self.call("chm_index_sort", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), self.Args["Np"],
+                  gs, ptr(d["indx_in_cell"]), ptr(d["cell_offset"]), ptr(d["sort_indx"]),
+                  ptr(d["Np_stay"]), ptr(ws), ws.numel())
+        dev_np_stay = d["Np_stay"]
+        self.Args.set_lazy("Np_stay", lambda: int(dev_np_stay.item()))
+
+    def sort_flags(self):
+        """Sticky sort error flags (bit 0: a cell exceeded 16384 particles)."""
+        return 0 if self._sort_ws is None else int(self._sort_flags.item())
+
+    # ---------------------------------------------------------------- align
+    def align_and_damp(self, comps_align):
+        """particles_methods_cl.py:263-293: keep the Np_stay particles of the
+        valid cells, in sorted order (used by the moving window)."""
+        Np_stay = self.Args["Np_stay"]
+        idx = self.DataDev["sort_indx"][:Np_stay].long()
+        for comp in comps_align:
+            self.DataDev[comp] = self.DataDev[comp].index_select(0, idx).contiguous()
+        self.reset_num_parts()
+        self.flag_sorted = False
+
+    def free_mp(self):
+        pass
+
+    def free_added(self):
+        for arg in self._arg_names():
+            self.DataDev.pop(arg + "_new", None)
+
+    def host(self, comp):
+        """Host copy of a particle component (float64 numpy)."""
+        return self.DataDev[comp].detach().double().cpu().numpy()
+
+    def new_args(self, configs):
+        a = LazyArgs(configs)
+        return a

@@ -1,0 +1,138 @@
+"""PIC_loop: the step orchestration of chimeraCL (chimeraCL/pic_loop.py:30-142).
+
+The order of operations is the reference's.  Three adjacent reference calls
+are fused into one kernel each, with identical arithmetic:
+  * fb_transform(dir=0) of rho, J + fields_smooth -> filter applied in the
+    transform's last pass;
+  * the dN0 <- dN1 copy + field_grad -> one assembly kernel;
+  * push_fields + damp_fields -> one row kernel (PSATD, then the damping
+    round trip through (kr, x) space).
+restore_B_fb computes rot(G)/w^2 in one pass.  With ``graph=True`` the step
+is captured once into a HIP graph (after one eager step) and replayed; the
+step then makes no host synchronisation and no allocation.
+
+Timing (timit=True) uses HIP events around each phase, keyed by the
+reference's loop_steps names.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+loop_steps = ["frame", "push-x", "sort", "depose", "transform", "smooth", "data_copy",
+              "grad", "push-eb", "damp-eb", "restore_B", "gather + push-p"]
+
+
+class PIC_loop:
+    def __init__(self, solvers=[], species=[], frames=[], diags=[], timit=False, graph=False):
+        self.solvers = solvers
+        self.mainsolver = self.solvers[0]
+        self.species = species
+        self.frames = frames
+        self.diags = diags
+        self.timit = timit
+        self.it = 0
+        self.graph = graph
+        self._graph = None
+        self.Timer = {s: 0.0 for s in loop_steps}
+        self._ev = []
+
+    # ---------------------------------------------------------------- timing
+    def timer_start(self):
+        if self.timit:
+            self._t0 = torch.cuda.Event(enable_timing=True)
+            self._t0.record()
+
+    def timer_record(self, name):
+        if self.timit:
+            t1 = torch.cuda.Event(enable_timing=True)
+            t1.record()
+            self._ev.append((name, self._t0, t1))
+
+    def _flush_timers(self):
+        if self._ev:
+            torch.cuda.synchronize()
+            for name, a, b in self._ev:
+                self.Timer[name] += a.elapsed_time(b) * 1e-3
+            self._ev = []
+
+    # ---------------------------------------------------------------- step
+    def _body(self):
+        self.timer_start()
+        for frame in self.frames:
+            if np.mod(self.it, frame.Args["Steps"]) == 0:
+                frame.shift_grids(grids=self.solvers)
+                frame.inject_plasma(species=self.species, grid=self.mainsolver)
+        self.timer_record("frame")
+        for parts in self.species:
+            self.timer_start()
+            parts.push_coords(mode="half")
+            self.timer_record("push-x")
+            self.timer_start()
+            parts.sort_parts(grid=self.mainsolver)
+            self.timer_record("sort")
+        self.timer_start()
+        for solver in self.solvers:
+            solver.depose_currents(species=self.species)
+        self.timer_record("depose")
+        for parts in self.species:
+            self.timer_start()
+            parts.push_coords(mode="half")
+            self.timer_record("push-x")
+            self.timer_start()
+            parts.sort_parts(grid=self.mainsolver)
+            self.timer_record("sort")
+        for solver in self.solvers:
+            self.timer_start()
+            solver.depose_charge(species=self.species)
+            self.timer_record("depose")
+            self.timer_start()
+            solver.fb_transform_smooth(scals=["rho"], vects=["J"])
+            self.timer_record("transform")
+            self.timer_start()
+            solver.field_grad("rho", "dN1", copy_to="dN0")
+            self.timer_record("grad")
+            self.timer_start()
+            solver.damp_edges(("E", "G"), advance_with=("J", "dN0", "dN1"))
+            self.timer_record("push-eb")
+            self.timer_start()
+            solver.restore_B_fb()
+            self.timer_record("restore_B")
+            self.timer_start()
+            solver.fb_transform(vects=["E", "B"], dir=1)
+            self.timer_record("transform")
+            self.timer_start()
+            solver.gather_and_push(species=self.species)
+            self.timer_record("gather + push-p")
+
+    def step(self):
+        for diag in self.diags:
+            diag.make_record(self.it)
+        if self.graph and self._graph is None and self.it >= 1 and not self.frames and not self.timit:
+            self._capture()
+        if self._graph is not None:
+            self._graph.replay()
+            for parts in self.species:
+                if parts.Args["Np"] > 0 and "Immobile" not in parts.Args:
+                    parts.Args.set_lazy("Np_stay", (lambda t: lambda: int(t.item()))(parts.DataDev["Np_stay"]))
+        else:
+            self._body()
+            self._flush_timers()
+        self.it += 1
+        return self.it
+
+    def _capture(self):
+        # one eager step has run, so every buffer exists; ions stay sorted
+        # (they are never pushed), electrons are sorted twice per step
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self._body()
+        torch.cuda.current_stream().wait_stream(s)
+        self._graph = g
+        # the capture ran no kernels: undo the half pushes' bookkeeping only
+        for parts in self.species:
+            parts.flag_sorted = parts.flag_sorted

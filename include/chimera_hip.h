@@ -1,0 +1,349 @@
+/*
+ * chimera_hip.h -- C ABI of libchimera_hip.so, the MI355X (gfx950) kernels of
+ * the chimeraCL quasi-cylindrical PSATD PIC step.
+ *
+ * Conventions
+ *   - Every entry point is asynchronous on the HIP stream it is given
+ *     (`stream`, a hipStream_t passed as void*; NULL = the null stream) and
+ *     returns 0 on success, a negative CHM_E* code for a bad argument, or a
+ *     positive hipError_t value when a launch failed.
+ *   - All array arguments are device pointers.  Struct arguments are host
+ *     pointers to plain structs holding device pointers and scalars.
+ *   - No entry point allocates memory: scratch space comes in as `ws`
+ *     (device pointer) with its size `ws_bytes`; each family has a
+ *     *_workspace_bytes() query.
+ *   - Suffix _f64: the plain FP64 form.  Suffix _f32: the mixed form's FP32
+ *     kernels (particles, gathers, deposits, FFT, pointwise spectral ops).
+ *     The mixed DHT product is selected in chm_spectral_t.dht_kind.
+ *
+ * Data layouts (the reference's, chimeraCL/grid.py:106-127,
+ * transformer.py:100-132):
+ *   particles     SoA [Np] (x, y, z, px, py, pz, w, g_inv)
+ *   real planes   [Nr][Nx], x fastest; mode 0 real, modes m>=1 complex
+ *                 interleaved (re, im); row 0 is the ghost row r = -dr/2
+ *   spectral      [Nr-1][Nx] complex per mode, kr-major, kx in FFT order
+ *   DHT tables    [Nr-1][Nr-1] real per mode
+ *
+ * Reference interfaces replaced (hightower8083/chimeraCL v0.0.1; the
+ * reference binds its OpenCL kernels from Python with PyOpenCL, so these are
+ * the methods of its *MethodsCL mixins and the kernels they launch):
+ *   chm_push_x_*            ParticleMethodsCL.push_coords
+ *                           (methods/particles_methods_cl.py:206-223,
+ *                            kernels/particles_generic.cl:129-153)
+ *   chm_index_sort_*        ParticleMethodsCL.index_sort
+ *                           (methods/particles_methods_cl.py:225-261,
+ *                            kernels/particles_generic.cl:88-126, 186-200)
+ *   chm_dens_profile_*      ParticleMethodsCL.dens_profile
+ *                           (methods/particles_methods_cl.py:179-204,
+ *                            kernels/particles_generic.cl:6-30)
+ *   chm_deposit_scalar_*    GridMethodsCL.depose_scalar
+ *                           (methods/grid_methods_cl.py:44-69,
+ *                            kernels/grid_deposit_m1.cl:20-152)
+ *   chm_deposit_vector_*    GridMethodsCL.depose_vector
+ *                           (methods/grid_methods_cl.py:71-95,
+ *                            kernels/grid_deposit_m1.cl:171-327)
+ *   chm_postproc_deposit_*  GridMethodsCL.postproc_depose_scalar/vector
+ *                           (methods/grid_methods_cl.py:98-151,
+ *                            kernels/grid_generic.cl:4-60)
+ *   chm_warp_axis_*         GridMethodsCL.preproc_project_vec
+ *                           (methods/grid_methods_cl.py:153-166,
+ *                            kernels/grid_generic.cl:63-86)
+ *   chm_gather_push_*       GridMethodsCL._gather_and_push
+ *                           (methods/grid_methods_cl.py:168-192,
+ *                            kernels/grid_deposit_m1.cl:330-511)
+ *   chm_transform_*         TransformerMethodsCL.transform_field, full and
+ *                           half modes (methods/transformer_methods_cl.py:
+ *                           38-64, 290-455; Reikna FFT and MatrixMul)
+ *   chm_mult_table_*        TransformerMethodsCL.fields_smooth and
+ *                           field_poiss_vec (transformer_methods_cl.py:66-85)
+ *   chm_field_grad_*        TransformerMethodsCL.field_grad (:87-133)
+ *   chm_field_div_*         TransformerMethodsCL.field_div (:135-183)
+ *   chm_field_rot_*         TransformerMethodsCL.field_rot (:185-263)
+ *   chm_advance_fields_*    SolverMethodsCL.advance_fields
+ *                           (methods/solver_methods_cl.py:43-62,
+ *                            kernels/solver_ms_pic.cl:57-143)
+ *   chm_damp_fields_*       Solver.damp_fields (solver.py:32-35:
+ *                           half transforms + profile_edges,
+ *                           solver_methods_cl.py:64-83)
+ *   chm_build_tables_f64    Transformer._make_spectral_axes / _make_DHT
+ *                           (transformer.py:28-97), Solver._make_ms_coefficients
+ *                           (solver.py:41-52), _init_field_damping
+ *                           (solver_methods_cl.py:32-41), get_phase_*
+ *                           (kernels/transformer_generic.cl:28-55)
+ *   chm_laser_*             add_gausian_pulse (laser.py:3-37)
+ */
+#ifndef CHIMERA_HIP_H
+#define CHIMERA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHM_MAXM 4 /* highest azimuthal mode supported (modes 0..CHM_MAXM) */
+
+/* Particle-grid geometry, from Grid.Args (grid.py:68-103). */
+typedef struct chm_grid {
+  int32_t Nx, Nr, M, pad_;
+  double xmin, dx_inv, rmin, dr_inv;
+} chm_grid_t;
+
+/* Real-space planes of one field component: m0 is [Nr][Nx] real, mc[m-1] is
+ * [Nr][Nx] complex for m = 1..M.  Element type: double (_f64) or float (_f32). */
+typedef struct chm_planes {
+  void *m0;
+  void *mc[CHM_MAXM];
+} chm_planes_t;
+
+/* Spectral planes of one field component: m[m] is [Nr-1][Nx] complex. */
+typedef struct chm_fb {
+  void *m[CHM_MAXM + 1];
+} chm_fb_t;
+
+/* DHT product used by the _f32 (mixed) transforms. */
+#define CHM_DHT_F32 0     /* FP32 MFMA product, FP32 accumulation            */
+#define CHM_DHT_BF16X3 1  /* split-BF16 3-term MFMA product, FP32 accumulate */
+
+/* Spectral tables (device pointers).  For _f64 every table is double; for
+ * _f32 every table is float, and the DHT matrices are additionally given as
+ * split-BF16 pairs (hi, lo) [Nk][Nk] each when dht_kind = CHM_DHT_BF16X3. */
+typedef struct chm_spectral {
+  int32_t Nx, Nr, M, ndamp; /* ndamp = 2*DampCells, 0 = no damping table   */
+  int32_t dht_kind, pad_;
+  double dt_inv;
+  const void *kx;          /* [Nx]                                         */
+  const void *phase_minus; /* [Nx] complex exp(-i kx Xmin)                 */
+  const void *phase_plus;  /* [Nx] complex exp(+i kx Xmin)                 */
+  const void *damp;        /* [ndamp]                                      */
+  const void *twiddle;     /* [Nx] complex exp(-2 pi i t / Nx)             */
+  const void *DHT[CHM_MAXM + 1];
+  const void *DHT_inv[CHM_MAXM + 1];
+  const void *dDp[CHM_MAXM + 1];
+  const void *dDm[CHM_MAXM + 1];
+  const void *DHT_lo[CHM_MAXM + 1];     /* bf16 low parts (BF16X3 only)  */
+  const void *DHT_inv_lo[CHM_MAXM + 1];
+  const void *dDp_lo[CHM_MAXM + 1];
+  const void *dDm_lo[CHM_MAXM + 1];
+  const void *c1[CHM_MAXM + 1];     /* cos(w dt)    [Nk][Nx]              */
+  const void *c2[CHM_MAXM + 1];     /* w sin(w dt)  [Nk][Nx]              */
+  const void *c3[CHM_MAXM + 1];     /* 1/w^2        [Nk][Nx] (= Poisson)  */
+  const void *smooth[CHM_MAXM + 1]; /* smoothing filter [Nk][Nx]          */
+} chm_spectral_t;
+
+/* Outputs of chm_build_tables_f64 (device pointers, all double). */
+typedef struct chm_tables_out {
+  double *kx;                /* [Nx]                                   */
+  double *kr;                /* [M+2][Nk]: kr_m for m = 0..M+1         */
+  double *phase_minus;       /* [Nx] complex                           */
+  double *phase_plus;        /* [Nx] complex                           */
+  double *damp;              /* [2*damp_cells] (may be NULL if 0)      */
+  double *twiddle;           /* [Nx] complex exp(-2 pi i t / Nx)       */
+  double *w[CHM_MAXM + 1];   /* [Nk][Nx]                               */
+  double *c1[CHM_MAXM + 1];
+  double *c2[CHM_MAXM + 1];
+  double *c3[CHM_MAXM + 1];  /* 1/w^2, also Poiss_m                    */
+  double *smooth[CHM_MAXM + 1];
+  double *DHT[CHM_MAXM + 1]; /* [Nk][Nk]                               */
+  double *DHT_inv[CHM_MAXM + 1];
+  double *dDp[CHM_MAXM + 1];
+  double *dDm[CHM_MAXM + 1];
+} chm_tables_out_t;
+
+/* ---------------------------------------------------------------- particles */
+int chm_push_x_f64(double *x, double *y, double *z, const double *px,
+                   const double *py, const double *pz, const double *g_inv,
+                   double dt, int64_t Np, void *stream);
+int chm_push_x_f32(float *x, float *y, float *z, const float *px,
+                   const float *py, const float *pz, const float *g_inv,
+                   float dt, int64_t Np, void *stream);
+
+/* Cell keys, histogram, offsets and the stable sort of index_sort.
+ * nbins = (Nx-1)*(Nr-1)+1 (last bin = particles outside the valid cells).
+ * Outputs: indx_in_cell [Np] (cell key), cell_offset [nbins+1]
+ * (cell_offset[c] = first sorted slot of bin c, cell_offset[nbins] = Np),
+ * sort_indx [Np] (particle indices ordered by key, ascending index within a
+ * key: numpy.argsort(key, kind='stable')), np_stay [1] (= cell_offset[nbins-1],
+ * the reference's Np_stay, kept on the device). */
+size_t chm_sort_workspace_bytes(int64_t Np, int64_t nbins);
+int chm_index_sort_f64(const double *x, const double *y, const double *z,
+                       int64_t Np, const chm_grid_t *g, uint32_t *indx_in_cell,
+                       uint32_t *cell_offset, uint32_t *sort_indx,
+                       uint32_t *np_stay, void *ws, size_t ws_bytes,
+                       void *stream);
+int chm_index_sort_f32(const float *x, const float *y, const float *z,
+                       int64_t Np, const chm_grid_t *g, uint32_t *indx_in_cell,
+                       uint32_t *cell_offset, uint32_t *sort_indx,
+                       uint32_t *np_stay, void *ws, size_t ws_bytes,
+                       void *stream);
+/* Byte offset, inside the sort workspace, of the sticky uint32 error flags
+ * (bit 0: a valid cell held more particles than the in-cell sort handles,
+ * 16384; its slots were left in arrival order). */
+size_t chm_sort_flags_offset(int64_t Np, int64_t nbins);
+
+int chm_dens_profile_f64(const double *x, double *w, int64_t Np,
+                         const double *xx, const double *ff,
+                         const double *dxm1, int32_t nknots, void *stream);
+int chm_dens_profile_f32(const float *x, float *w, int64_t Np,
+                         const double *xx, const double *ff,
+                         const double *dxm1, int32_t nknots, void *stream);
+
+/* ------------------------------------------------------------ grid <-> particles */
+size_t chm_deposit_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M,
+                                   int32_t vector);
+/* Adds the species' charge density (w*charge, CIC, modes 0..M) into rho. */
+int chm_deposit_scalar_f64(const uint32_t *sort_indx,
+                           const uint32_t *cell_offset, const double *x,
+                           const double *y, const double *z, const double *w,
+                           int32_t charge, const chm_grid_t *g,
+                           const chm_planes_t *rho, void *ws, size_t ws_bytes,
+                           void *stream);
+int chm_deposit_scalar_f32(const uint32_t *sort_indx,
+                           const uint32_t *cell_offset, const float *x,
+                           const float *y, const float *z, const float *w,
+                           int32_t charge, const chm_grid_t *g,
+                           const chm_planes_t *rho, void *ws, size_t ws_bytes,
+                           void *stream);
+/* Adds the species' current (w*g_inv*charge*u) into J[0..2]. */
+int chm_deposit_vector_f64(const uint32_t *sort_indx,
+                           const uint32_t *cell_offset, const double *x,
+                           const double *y, const double *z, const double *ux,
+                           const double *uy, const double *uz,
+                           const double *g_inv, const double *w, int32_t charge,
+                           const chm_grid_t *g, const chm_planes_t *J,
+                           void *ws, size_t ws_bytes, void *stream);
+int chm_deposit_vector_f32(const uint32_t *sort_indx,
+                           const uint32_t *cell_offset, const float *x,
+                           const float *y, const float *z, const float *ux,
+                           const float *uy, const float *uz,
+                           const float *g_inv, const float *w, int32_t charge,
+                           const chm_grid_t *g, const chm_planes_t *J,
+                           void *ws, size_t ws_bytes, void *stream);
+/* treat_axis (row1 -= row0) then *= dV_inv[ir], for nflds fields. */
+int chm_postproc_deposit_f64(const chm_planes_t *flds, int32_t nflds,
+                             const double *dV_inv, int32_t Nx, int32_t Nr,
+                             int32_t M, void *stream);
+int chm_postproc_deposit_f32(const chm_planes_t *flds, int32_t nflds,
+                             const double *dV_inv, int32_t Nx, int32_t Nr,
+                             int32_t M, void *stream);
+/* Ghost row: row0 = row1 (m=0), row0 = -row1 (m>=1), for nflds fields. */
+int chm_warp_axis_f64(const chm_planes_t *flds, int32_t nflds, int32_t Nx,
+                      int32_t Nr, int32_t M, void *stream);
+int chm_warp_axis_f32(const chm_planes_t *flds, int32_t nflds, int32_t Nx,
+                      int32_t Nr, int32_t M, void *stream);
+/* Field gather and relativistic Boris push of every particle p with
+ * p < *np_stay inside a valid cell (the set the reference's per-slot launch
+ * visits).  factor_push = Args['FactorPush'] = 2*pi*dt*q/m. */
+int chm_gather_push_f64(const double *x, const double *y, const double *z,
+                        double *px, double *py, double *pz, double *g_inv,
+                        int64_t Np, const uint32_t *np_stay,
+                        double factor_push, const chm_grid_t *g,
+                        const chm_planes_t *E, const chm_planes_t *B,
+                        void *stream);
+int chm_gather_push_f32(const float *x, const float *y, const float *z,
+                        float *px, float *py, float *pz, float *g_inv,
+                        int64_t Np, const uint32_t *np_stay, float factor_push,
+                        const chm_grid_t *g, const chm_planes_t *E,
+                        const chm_planes_t *B, void *stream);
+
+/* ------------------------------------------------------------- transforms */
+size_t chm_transform_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M,
+                                     int32_t nflds, int32_t elem_bytes);
+/* dir 0: real -> spectral, dir 1: spectral -> real (rows 1..Nr-1).
+ * mode 0: full (DHT + FFT_x + phase); mode 1: half (FFT_x + phase only).
+ * post_smooth (dir 0 only): if nonzero, multiply the result by sp->smooth. */
+int chm_transform_f64(const chm_spectral_t *sp, const chm_planes_t *real,
+                      const chm_fb_t *fb, int32_t nflds, int32_t dir,
+                      int32_t mode, int32_t post_smooth, void *ws,
+                      size_t ws_bytes, void *stream);
+int chm_transform_f32(const chm_spectral_t *sp, const chm_planes_t *real,
+                      const chm_fb_t *fb, int32_t nflds, int32_t dir,
+                      int32_t mode, int32_t post_smooth, void *ws,
+                      size_t ws_bytes, void *stream);
+/* which 0: *= smooth (fields_smooth); which 1: *= 1/w^2 (field_poiss_*). */
+int chm_mult_table_f64(const chm_spectral_t *sp, const chm_fb_t *flds,
+                       int32_t nflds, int32_t which, void *stream);
+int chm_mult_table_f32(const chm_spectral_t *sp, const chm_fb_t *flds,
+                       int32_t nflds, int32_t which, void *stream);
+/* out[3] = grad(scl).  If prev is not NULL, the old out[3] is first copied to
+ * prev[3] (the dN0 <- dN1 copy of pic_loop.py:106-112). */
+size_t chm_spectral_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M,
+                                    int32_t elem_bytes);
+int chm_field_grad_f64(const chm_spectral_t *sp, const chm_fb_t *scl,
+                       const chm_fb_t *out, const chm_fb_t *prev, void *ws,
+                       size_t ws_bytes, void *stream);
+int chm_field_grad_f32(const chm_spectral_t *sp, const chm_fb_t *scl,
+                       const chm_fb_t *out, const chm_fb_t *prev, void *ws,
+                       size_t ws_bytes, void *stream);
+int chm_field_div_f64(const chm_spectral_t *sp, const chm_fb_t *vec,
+                      const chm_fb_t *out, void *ws, size_t ws_bytes,
+                      void *stream);
+int chm_field_div_f32(const chm_spectral_t *sp, const chm_fb_t *vec,
+                      const chm_fb_t *out, void *ws, size_t ws_bytes,
+                      void *stream);
+/* out[3] = rot(in[3]); poisson != 0 also multiplies by 1/w^2 (restore_B_fb). */
+int chm_field_rot_f64(const chm_spectral_t *sp, const chm_fb_t *in,
+                      const chm_fb_t *out, int32_t poisson, void *ws,
+                      size_t ws_bytes, void *stream);
+int chm_field_rot_f32(const chm_spectral_t *sp, const chm_fb_t *in,
+                      const chm_fb_t *out, int32_t poisson, void *ws,
+                      size_t ws_bytes, void *stream);
+
+/* ----------------------------------------------------------------- solver */
+int chm_advance_fields_f64(const chm_spectral_t *sp, const chm_fb_t *E,
+                           const chm_fb_t *G, const chm_fb_t *J,
+                           const chm_fb_t *dN0, const chm_fb_t *dN1,
+                           void *stream);
+int chm_advance_fields_f32(const chm_spectral_t *sp, const chm_fb_t *E,
+                           const chm_fb_t *G, const chm_fb_t *J,
+                           const chm_fb_t *dN0, const chm_fb_t *dN1,
+                           void *stream);
+/* Edge damping of E[3] and G[3] in (kr, x) space, in place in spectral
+ * space; with advance != 0 the PSATD advance is applied first in the same
+ * kernel (J, dN0, dN1 then required). */
+int chm_damp_fields_f64(const chm_spectral_t *sp, const chm_fb_t *E,
+                        const chm_fb_t *G, int32_t advance, const chm_fb_t *J,
+                        const chm_fb_t *dN0, const chm_fb_t *dN1,
+                        void *stream);
+int chm_damp_fields_f32(const chm_spectral_t *sp, const chm_fb_t *E,
+                        const chm_fb_t *G, int32_t advance, const chm_fb_t *J,
+                        const chm_fb_t *dN0, const chm_fb_t *dN1,
+                        void *stream);
+
+/* ----------------------------------------------------------------- tables */
+size_t chm_tables_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M);
+/* Builds every spectral table in FP64 on the GPU: Bessel roots (McMahon
+ * start + Newton), J_m matrices, their inverse (Gauss-Jordan with partial
+ * pivoting), dDHT+-, kx, w, PSATD coefficients, smoothing filter, phases and
+ * the damping profile. */
+int chm_build_tables_f64(int32_t Nx, int32_t Nr, int32_t M, double dx,
+                         double dr, double dt, double xmin, int32_t damp_cells,
+                         const chm_tables_out_t *out, void *ws,
+                         size_t ws_bytes, void *stream);
+/* Plain FP64 product C[M x N] = A[M x K] @ B[K x N] on the matrix cores
+ * (row-major, contiguous); used for the dDHT tables and by the tests. */
+int chm_gemm_f64(const double *A, const double *B, double *C, int32_t M,
+                 int32_t K, int32_t N, void *stream);
+/* Converts n doubles to float (round to nearest) and, if hi/lo are not NULL,
+ * to a split-BF16 pair with hi + lo ~= src. */
+int chm_convert_table(const double *src, int64_t n, float *f32,
+                      uint16_t *bf16_hi, uint16_t *bf16_lo, void *stream);
+
+/* ------------------------------------------------------------------ laser */
+/* Ez_m0[1:] = a0 sin(k0 (x - x0)) exp(-(x-x0)^2/Lx^2 - r^2/R^2) inside the
+ * 3.5 R, 3.5 Lx cut-offs (laser.py:17-19); row 0 untouched. */
+int chm_laser_pulse_f64(double *Ez_m0, int32_t Nx, int32_t Nr, double xmin,
+                        double dx, double rmin, double dr, double k0,
+                        double a0, double x0, double Lx, double R,
+                        void *stream);
+/* Spectral part of laser.py:24-34 on mode 0: G = -i w sign(kx) E, then the
+ * vacuum propagation over X_focus; writes E and G in place. */
+int chm_laser_spectral_f64(const chm_spectral_t *sp, const double *w_m0,
+                           void *Ez_fb_m0, void *Gz_fb_m0, double X_focus,
+                           void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

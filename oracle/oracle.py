@@ -1,0 +1,592 @@
+"""CPU FP64 oracle of the chimeraCL PIC step.
+
+TEST INFRASTRUCTURE ONLY.  This module restates the reference algorithm
+(hightower8083/chimeraCL v0.0.1) on the CPU so that the HIP path can be
+checked against it.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker or
+as the timed CPU baseline; the product path (``chimeracl_amd``) never imports
+or calls anything under ``oracle/``.
+
+What it restates (reference file:line in each docstring):
+  * geometry and derived arguments         grid.py:68-103, particles.py:53-100
+  * spectral axes, DHT matrices           transformer.py:28-97
+  * PSATD coefficient tables              solver.py:41-52
+  * damping profile                       solver_methods_cl.py:32-41
+  * transforms (full and half)            transformer_methods_cl.py:38-64, 290-455
+  * get_m1, grad, div, rot, Poisson, smoothing
+                                          transformer_methods_cl.py:66-288,
+                                          transformer_generic.cl:3-24
+  * PSATD advance, edge damping            solver_ms_pic.cl:5-143
+  * particle push, cell keys, sort, deposit, axis post-processing, gather
+    and Boris push                        particles_oracle.c (C, IEEE, no FMA),
+                                          grid_generic.cl:4-86
+  * the step order                        pic_loop.py:57-142
+  * the laser initialiser                 laser.py:3-37
+
+Numerics: plain IEEE FP64.  The particle loops are C built without FMA
+contraction; the transforms use numpy.fft (the numpy inverse-FFT
+normalisation 1/Nx, which the reference's round-trip test implies,
+examples/test_transformer.py:35-43) and numpy matmul.  The reference builds
+its kernels with -cl-fast-relaxed-math (generic_methods_cl.py:15); this
+oracle is the exact-arithmetic refinement of that.
+
+The in-cell particle order is the canonical stable one (ascending particle
+index).  The reference's atomic scatter leaves it nondeterministic
+(particles_generic.cl:186-200), so any fixed order is a valid refinement.
+
+Parity pins (tests/test_oracle_*.py): every function here is checked against
+something the reference or mathematics fixes -- closed forms, the two
+known-answer scripts of the reference (round-trip identity and laser group
+velocity), committed scipy/mpmath fixtures for the Bessel tables,
+numpy.fft / brute-force loops on tiny inputs, sum rules and single-particle
+closed forms for the deposit, and textbook Boris identities.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+from scipy.special import jn_zeros, jv
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBNAME = os.path.join(_HERE, "liboracle_particles.so")
+_lib = None
+
+
+def build():
+    """Compile the C part of the oracle (make -C oracle)."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBNAME):
+            build()
+        L = ctypes.CDLL(_LIBNAME)
+        P = ctypes.c_void_p
+        i64, i32, f64 = ctypes.c_int64, ctypes.c_int32, ctypes.c_double
+        L.orc_push_xyz.argtypes = [P, P, P, P, P, P, P, f64, i64]
+        L.orc_cell_keys.argtypes = [P, P, P, i64, i32, f64, f64, i32, f64, f64, P]
+        L.orc_stable_sort.argtypes = [P, i64, i64, P, P]
+        L.orc_deposit_scalar.argtypes = [P, P, P, P, P, P, i32, i32, f64, f64,
+                                         i32, f64, f64, i32, P, P]
+        L.orc_deposit_vector.argtypes = [P, P, P, P, P, P, P, P, P, P, i32, i32,
+                                         f64, f64, i32, f64, f64, i32, P, P]
+        L.orc_gather_push.argtypes = [P, P, P, P, P, P, P, P, i64, i64, f64, i32,
+                                      f64, f64, i32, f64, f64, i32, P, P, P, P]
+        L.orc_profile_by_interpolant.argtypes = [P, P, i64, P, P, P, i32]
+        L.orc_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# --------------------------------------------------------------------------
+# geometry
+# --------------------------------------------------------------------------
+def grid_args(cfg):
+    """Grid._process_configs, grid.py:68-103."""
+    a = dict(cfg)
+    a.setdefault("M", 0)
+    a["dx"] = (a["Xmax"] - a["Xmin"]) / (a["Nx"] - 1)
+    a["dx_inv"] = 1.0 / a["dx"]
+    a["dr"] = a["Rmax"] / (a["Nr"] - 1.5)
+    a["dr_inv"] = 1.0 / a["dr"]
+    if "dt" not in a:
+        a["dt"] = a["dx"]
+    a["dt_inv"] = 1.0 / a["dt"]
+    a["Xgrid"] = a["Xmin"] + a["dx"] * np.arange(a["Nx"])
+    a["Rmin"] = -0.5 * a["dr"]
+    a["Rgrid"] = a["Rmin"] + a["dr"] * np.arange(a["Nr"])
+    a["Rmax"] = a["Rgrid"].max()
+    with np.errstate(divide="ignore"):
+        a["dV_inv"] = (a["Rgrid"] > 0) / (2 * np.pi * a["dx"] * a["dr"] * a["Rgrid"])
+    a["NxNr"] = a["Nr"] * a["Nx"]
+    a["Nxm1Nrm1"] = (a["Nr"] - 1) * (a["Nx"] - 1)
+    a["NxNrm1"] = (a["Nr"] - 1) * a["Nx"]
+    return a
+
+
+def particle_args(cfg):
+    """Particles._process_configs, particles.py:53-100."""
+    a = dict(cfg)
+    a.setdefault("dt", 1.0)
+    a.setdefault("dx", 1.0)
+    a.setdefault("dr", 1.0)
+    a.setdefault("charge", -1.0)
+    a.setdefault("mass", 1.0)
+    a.setdefault("dens", 1.0)
+    a["dt_2"] = 0.5 * a["dt"]
+    if "Nppc" in a:
+        nppc = np.array(a["Nppc"], dtype=np.uint32)
+        a["w0"] = 2 * np.pi * a["dx"] * a["dr"] * a["dens"] / np.prod(nppc)
+    a["FactorPush"] = 2 * np.pi * a["dt"] * a["charge"] / a["mass"]
+    return a
+
+
+def spectral_axes(ga):
+    """Transformer._make_spectral_axes, transformer.py:28-63."""
+    Nx, Nr, M, dx, dr = ga["Nx"], ga["Nr"], ga["M"], ga["dx"], ga["dr"]
+    kx = 2 * np.pi * np.fft.fftfreq(Nx, dx)
+    R_period = ga["Rgrid"][-1] + 0.5 * dr
+    kr = [jn_zeros(m, Nr - 1) / R_period for m in range(M + 2)]
+    w = [np.sqrt(kx[None, :] ** 2 + kr[m][:, None] ** 2) for m in range(M + 1)]
+    poiss = [1.0 / w[m] ** 2 for m in range(M + 1)]
+    smooth = [(1 - np.sin(0.5 * np.pi * kx[None, :] / kx.max()) ** 2)
+              * (1 - np.sin(0.5 * np.pi * kr[m][:, None] / kr[m].max()) ** 2)
+              for m in range(M + 1)]
+    return dict(kx=kx, kr=kr, w=w, Poiss=poiss, Smooth=smooth)
+
+
+def dht_matrices(ga):
+    """Transformer._make_DHT, transformer.py:65-97.
+
+    jn_zeros(-1, n) is the zeros of J_1 (scipy takes |n|)."""
+    Nr, M, dr = ga["Nr"], ga["M"], ga["dr"]
+    Rg = ga["Rgrid"][1:, None]
+    R_period = Rg[-1] + 0.5 * dr
+    out = dict(DHT=[], DHT_inv=[], dDp=[], dDm=[])
+    for m in range(M + 1):
+        kr0 = jn_zeros(m, Nr - 1) / R_period
+        krp = jn_zeros(m + 1, Nr - 1) / R_period
+        krm = jn_zeros(abs(m - 1), Nr - 1) / R_period
+        inv = jv(m, Rg * kr0)
+        fwd = np.linalg.pinv(inv)
+        out["DHT_inv"].append(inv)
+        out["DHT"].append(fwd)
+        out["dDp"].append(fwd.dot(0.5 * krp * jv(m, Rg * krp)))
+        out["dDm"].append(fwd.dot(0.5 * krm * jv(m, Rg * krm)))
+    return out
+
+
+def ms_coefficients(w, dt):
+    """Solver._make_ms_coefficients, solver.py:41-52."""
+    return np.cos(w * dt), np.sin(w * dt) * w, 1.0 / w ** 2
+
+
+def damp_profile(N):
+    """SolverMethodsCL._init_field_damping, solver_methods_cl.py:32-41."""
+    z = np.arange(2 * N)
+    z_shft = 3.0 * (z - N + 1) / (N + 1)
+    return ((z < 4.0 * N / 3) * (z >= N) * np.sin(0.5 * np.pi * z_shft) ** 2
+            + (z >= 4.0 * N / 3))
+
+
+def get_m1(f):
+    """get_m1, transformer_generic.cl:3-24: F_{-1}[ir,ix] = -conj(F_1[ir,-ix])."""
+    Nx = f.shape[1]
+    idx = (Nx - np.arange(Nx)) % Nx
+    g = f[:, idx]
+    return -g.real + 1j * g.imag
+
+
+# --------------------------------------------------------------------------
+# particles
+# --------------------------------------------------------------------------
+class OracleParticles:
+    """Particles (particles.py) with ParticleMethodsCL semantics."""
+
+    def __init__(self, cfg):
+        self.Args = particle_args(cfg)
+        self.immobile = "Immobile" in self.Args
+        self.data = {}
+        self.Args["Np"] = 0
+        self.Args["Np_stay"] = 0
+        self.flag_sorted = False
+
+    def add_particles(self, arrays):
+        """add_new_particles(source=...), particles_methods_cl.py:40-61."""
+        keys = ["x", "y", "z", "w"] if self.immobile else \
+            ["x", "y", "z", "px", "py", "pz", "w", "g_inv"]
+        for k in keys:
+            new = np.ascontiguousarray(arrays[k], dtype=np.float64)
+            old = self.data.get(k)
+            self.data[k] = new.copy() if old is None else np.concatenate([old, new])
+        self.Args["Np"] = self.data["x"].size
+        self.Args["Np_stay"] = self.Args["Np"]
+        self.flag_sorted = False
+
+    def dens_profile(self, x_prf, f_prf):
+        """dens_profile + profile_by_interpolant, particles_methods_cl.py:179-204,
+        particles_generic.cl:6-30 (constant extension outside the knots)."""
+        xx = np.ascontiguousarray(x_prf, dtype=np.float64)
+        ff = np.ascontiguousarray(f_prf, dtype=np.float64)
+        dxm1 = np.ascontiguousarray(1.0 / (xx[1:] - xx[:-1]))
+        d = self.data
+        lib().orc_profile_by_interpolant(_p(d["x"]), _p(d["w"]), d["x"].size,
+                                         _p(xx), _p(ff), _p(dxm1), xx.size)
+
+    def push_coords(self, mode="half"):
+        """push_coords, particles_methods_cl.py:206-223."""
+        if self.Args["Np"] == 0 or self.immobile:
+            return
+        dt = self.Args["dt_2"] if mode == "half" else self.Args["dt"]
+        d = self.data
+        lib().orc_push_xyz(_p(d["x"]), _p(d["y"]), _p(d["z"]), _p(d["px"]),
+                           _p(d["py"]), _p(d["pz"]), _p(d["g_inv"]), dt,
+                           self.Args["Np"])
+        self.flag_sorted = False
+
+    def sort_parts(self, grid):
+        """sort_parts, particles.py:23-29 -> index_sort, particles_methods_cl.py:225-261."""
+        if self.Args["Np"] == 0:
+            self.flag_sorted = True
+        if not self.flag_sorted:
+            self.index_sort(grid)
+            self.flag_sorted = True
+
+    def index_sort(self, grid):
+        g, d, Np = grid.Args, self.data, self.Args["Np"]
+        key = np.empty(Np, np.uint32)
+        lib().orc_cell_keys(_p(d["x"]), _p(d["y"]), _p(d["z"]), Np, g["Nx"],
+                            g["Xmin"], g["dx_inv"], g["Nr"], g["Rmin"],
+                            g["dr_inv"], _p(key))
+        nbins = g["Nxm1Nrm1"] + 1
+        off = np.empty(nbins + 1, np.uint32)
+        srt = np.empty(Np, np.uint32)
+        lib().orc_stable_sort(_p(key), Np, nbins, _p(off), _p(srt))
+        d["indx_in_cell"] = key
+        d["cell_offset"] = off
+        d["sort_indx"] = srt
+        self.Args["Np_stay"] = int(off[nbins - 1])
+
+
+# --------------------------------------------------------------------------
+# grid + transformer + solver
+# --------------------------------------------------------------------------
+class OracleSolver:
+    """Solver(Grid, Transformer, SolverMethodsCL), solver.py:9-68."""
+
+    comps = ("x", "y", "z")
+
+    def __init__(self, cfg):
+        self.Args = a = grid_args(cfg)
+        Nx, Nr, M = a["Nx"], a["Nr"], a["M"]
+        self.M = M
+        sp = spectral_axes(a)
+        a.update(kx=sp["kx"])
+        self.sp = sp
+        self.dht = dht_matrices(a)
+        self.coef = [ms_coefficients(sp["w"][m], a["dt"]) for m in range(M + 1)]
+        self.damp = damp_profile(int(a["DampCells"])) if "DampCells" in a else None
+        self.f = {}
+        for fld in ["E", "B", "J", "G"]:
+            for c in self.comps:
+                self._alloc_real(fld + c)
+        self._alloc_real("rho")
+        for fld in ["E", "B", "G", "J", "dN0", "dN1"]:
+            for c in self.comps:
+                for m in range(M + 1):
+                    self.f[f"{fld}{c}_fb_m{m}"] = np.zeros((Nr - 1, Nx), complex)
+        for m in range(M + 1):
+            self.f[f"rho_fb_m{m}"] = np.zeros((Nr - 1, Nx), complex)
+
+    def _alloc_real(self, name):
+        Nx, Nr = self.Args["Nx"], self.Args["Nr"]
+        self.f[name + "_m0"] = np.zeros((Nr, Nx))
+        for m in range(1, self.M + 1):
+            self.f[name + f"_m{m}"] = np.zeros((Nr, Nx), complex)
+
+    # ---- phases, transformer_generic.cl:28-55
+    def phase(self, dir):
+        arg = self.Args["Xmin"] * self.Args["kx"]
+        return np.cos(arg) + (1j if dir == 1 else -1j) * np.sin(arg)
+
+    # ---- transforms, transformer_methods_cl.py:38-64, 290-455
+    def transform_field(self, name, dir, mode="full"):
+        phs = self.phase(dir)[None, :]
+        f, M = self.f, self.M
+        for m in range(M + 1):
+            rs, fb = f"{name}_m{m}", f"{name}_fb_m{m}"
+            if dir == 0:
+                src = f[rs][1:]
+                if mode == "full":
+                    src = self.dht["DHT"][m] @ src
+                f[fb][:] = np.fft.fft(src.astype(complex), axis=1) * phs
+            else:
+                buf = np.fft.ifft(f[fb] * phs, axis=1)
+                if m == 0:
+                    buf = buf.real
+                if mode == "full":
+                    buf = self.dht["DHT_inv"][m] @ buf
+                f[rs][1:] = buf
+
+    def fb_transform(self, scals=(), vects=(), dir=0, mode="full"):
+        """Transformer.fb_transform, transformer.py:17-26."""
+        for s in scals:
+            self.transform_field(s, dir, mode)
+        for v in vects:
+            for c in self.comps:
+                self.transform_field(v + c, dir, mode)
+
+    # ---- spectral operators, transformer_methods_cl.py:66-263
+    def fields_smooth(self, flds):
+        for m in range(self.M + 1):
+            for fl in flds:
+                self.f[f"{fl}_fb_m{m}"] *= self.sp["Smooth"][m]
+
+    def field_poiss_vec(self, fld):
+        for m in range(self.M + 1):
+            for c in self.comps:
+                self.f[f"{fld}{c}_fb_m{m}"] *= self.sp["Poiss"][m]
+
+    def field_grad(self, scl, vec):
+        f, M, kx = self.f, self.M, self.Args["kx"][None, :]
+        mm1 = get_m1(f[f"{scl}_fb_m1"]) if M > 0 else None
+        for m in range(M + 1):
+            x = kx * (1j * f[f"{scl}_fb_m{m}"])
+            y = np.zeros_like(x)
+            z = np.zeros_like(x)
+            if m > 0 or M > 0:
+                src = f[f"{scl}_fb_m{m-1}"] if m > 0 else mm1
+                b = self.dht["dDm"][m] @ src
+                y += -1.0 * b
+                z += -1j * b
+                if m < M:
+                    b = self.dht["dDp"][m] @ f[f"{scl}_fb_m{m+1}"]
+                    y += b
+                    z += -1j * b
+            f[f"{vec}x_fb_m{m}"][:] = x
+            f[f"{vec}y_fb_m{m}"][:] = y
+            f[f"{vec}z_fb_m{m}"][:] = z
+
+    def field_div(self, vec):
+        """field_div, transformer_methods_cl.py:135-183 (returns the list of modes)."""
+        f, M, kx = self.f, self.M, self.Args["kx"][None, :]
+        if M > 0:
+            my = get_m1(f[f"{vec}y_fb_m1"])
+            mz = get_m1(f[f"{vec}z_fb_m1"])
+        out = []
+        for m in range(M + 1):
+            o = kx * (1j * f[f"{vec}x_fb_m{m}"])
+            if m > 0 or M > 0:
+                fy = f[f"{vec}y_fb_m{m-1}"] if m > 0 else my
+                fz = f[f"{vec}z_fb_m{m-1}"] if m > 0 else mz
+                o = o + self.dht["dDm"][m] @ (-1j * fz - 1.0 * fy)
+                if m < M:
+                    fy, fz = f[f"{vec}y_fb_m{m+1}"], f[f"{vec}z_fb_m{m+1}"]
+                    o = o + self.dht["dDp"][m] @ (-1j * fz + 1.0 * fy)
+            out.append(o)
+        return out
+
+    def field_rot(self, fin, fout):
+        f, M, kx = self.f, self.M, self.Args["kx"][None, :]
+        if M > 0:
+            mm1 = {c: get_m1(f[f"{fin}{c}_fb_m1"]) for c in self.comps}
+        for m in range(M + 1):
+            bx = np.zeros_like(f[f"{fin}x_fb_m{m}"])
+            by = kx * (-1j * f[f"{fin}z_fb_m{m}"])
+            bz = kx * (1j * f[f"{fin}y_fb_m{m}"])
+            if m > 0 or M > 0:
+                if m > 0:
+                    gx, gy, gz = (f[f"{fin}{c}_fb_m{m-1}"] for c in self.comps)
+                else:
+                    gx, gy, gz = mm1["x"], mm1["y"], mm1["z"]
+                bx = bx + self.dht["dDm"][m] @ (-1 * gz + 1j * gy)
+                b = self.dht["dDm"][m] @ gx
+                by = by + (-1j) * b
+                bz = bz + 1 * b
+                if m < M:
+                    gx, gy, gz = (f[f"{fin}{c}_fb_m{m+1}"] for c in self.comps)
+                    bx = bx + self.dht["dDp"][m] @ (1 * gz + 1j * gy)
+                    b = self.dht["dDp"][m] @ gx
+                    by = by + (-1j) * b
+                    bz = bz + (-1) * b
+            f[f"{fout}x_fb_m{m}"][:] = bx
+            f[f"{fout}y_fb_m{m}"][:] = by
+            f[f"{fout}z_fb_m{m}"][:] = bz
+
+    # ---- solver, solver.py:29-39 and solver_ms_pic.cl
+    def push_fields(self):
+        """advance_e_g_m, solver_ms_pic.cl:57-143, per mode."""
+        f, pi2, dti = self.f, 2 * np.pi, self.Args["dt_inv"]
+        for m in range(self.M + 1):
+            c1, c2, c3 = self.coef[m]
+            for c in self.comps:
+                e0 = f[f"E{c}_fb_m{m}"]
+                g0 = f[f"G{c}_fb_m{m}"]
+                j0 = f[f"J{c}_fb_m{m}"] * pi2
+                n0 = f[f"dN0{c}_fb_m{m}"] * pi2
+                n1 = f[f"dN1{c}_fb_m{m}"] * pi2
+                e1 = (c1 * e0 + c2 * c3 * (g0 - j0)
+                      + c3 * (c1 * n0 - n1 - (n0 - n1) * dti * c2 * c3))
+                g1 = (-c2 * e0 + c1 * (g0 - j0) + j0
+                      + c3 * (dti * (1.0 - c1) * (n0 - n1) - c2 * n0))
+                e0[:] = e1
+                g0[:] = g1
+
+    def profile_edges(self, flds):
+        """profile_edges_d/c, solver_ms_pic.cl:5-55 (whole [Nr,Nx] plane)."""
+        Nx, prof = self.Args["Nx"], self.damp
+        Nf = prof.size
+        fac = np.ones(Nx)
+        ix = np.arange(Nx)
+        lo = ix < Nf
+        fac[lo] *= prof[ix[lo]]
+        hi = ix > Nx - Nf
+        fac[hi] *= prof[Nx - ix[hi]]
+        for fl in flds:
+            for c in self.comps:
+                for m in range(self.M + 1):
+                    self.f[f"{fl}{c}_m{m}"] *= fac[None, :]
+
+    def damp_fields(self):
+        """Solver.damp_fields, solver.py:32-35."""
+        self.fb_transform(vects=["E", "G"], dir=1, mode="half")
+        self.profile_edges(["E", "G"])
+        self.fb_transform(vects=["E", "G"], dir=0, mode="half")
+
+    def restore_B_fb(self):
+        self.field_rot("G", "B")
+        self.field_poiss_vec("B")
+
+    # ---- grid, grid.py:23-66 and grid_methods_cl.py
+    def _postproc(self, name):
+        """treat_axis then divide_by_dv, grid_generic.cl:4-60."""
+        dvi = self.Args["dV_inv"][:, None]
+        for m in range(self.M + 1):
+            a = self.f[f"{name}_m{m}"]
+            a[1] -= a[0]
+            a *= dvi
+
+    def _modes(self, name):
+        M = self.M
+        m0 = self.f[name + "_m0"]
+        mc = (np.stack([self.f[name + f"_m{m}"] for m in range(1, M + 1)])
+              if M > 0 else np.zeros((1, 1, 1), complex))
+        return m0, np.ascontiguousarray(mc)
+
+    def depose_charge(self, species):
+        a, M = self.Args, self.M
+        m0 = np.zeros((a["Nr"], a["Nx"]))
+        mc = np.zeros((max(M, 1), a["Nr"], a["Nx"]), complex)
+        for parts in species:
+            if parts.Args["Np"] <= 0:
+                continue
+            d = parts.data
+            lib().orc_deposit_scalar(_p(d["sort_indx"]), _p(d["cell_offset"]),
+                                     _p(d["x"]), _p(d["y"]), _p(d["z"]), _p(d["w"]),
+                                     int(parts.Args["charge"]), a["Nx"], a["Xmin"],
+                                     a["dx_inv"], a["Nr"], a["Rmin"], a["dr_inv"], M,
+                                     _p(m0), _p(mc))
+        self.f["rho_m0"][:] = m0
+        for m in range(1, M + 1):
+            self.f[f"rho_m{m}"][:] = mc[m - 1]
+        self._postproc("rho")
+
+    def depose_currents(self, species):
+        a, M = self.Args, self.M
+        m0 = np.zeros((3, a["Nr"], a["Nx"]))
+        mc = np.zeros((3, max(M, 1), a["Nr"], a["Nx"]), complex)
+        for parts in species:
+            if parts.immobile or parts.Args["Np"] <= 0:
+                continue
+            d = parts.data
+            lib().orc_deposit_vector(_p(d["sort_indx"]), _p(d["cell_offset"]),
+                                     _p(d["x"]), _p(d["y"]), _p(d["z"]),
+                                     _p(d["px"]), _p(d["py"]), _p(d["pz"]),
+                                     _p(d["g_inv"]), _p(d["w"]),
+                                     int(parts.Args["charge"]), a["Nx"], a["Xmin"],
+                                     a["dx_inv"], a["Nr"], a["Rmin"], a["dr_inv"], M,
+                                     _p(m0), _p(mc))
+        for k, c in enumerate(self.comps):
+            self.f[f"J{c}_m0"][:] = m0[k]
+            for m in range(1, M + 1):
+                self.f[f"J{c}_m{m}"][:] = mc[k, m - 1]
+            self._postproc("J" + c)
+
+    def preproc_project_vec(self, fld):
+        """warp_axis_m0_d / warp_axis_m1plus_c, grid_generic.cl:63-86."""
+        for c in self.comps:
+            a = self.f[f"{fld}{c}_m0"]
+            a[0] = a[1]
+            for m in range(1, self.M + 1):
+                a = self.f[f"{fld}{c}_m{m}"]
+                a[0] = -a[1]
+
+    def gather_and_push(self, species):
+        for fld in ("E", "B"):
+            self.preproc_project_vec(fld)
+        a, M = self.Args, self.M
+        E0 = np.ascontiguousarray(np.stack([self.f[f"E{c}_m0"] for c in self.comps]))
+        B0 = np.ascontiguousarray(np.stack([self.f[f"B{c}_m0"] for c in self.comps]))
+        if M > 0:
+            Ec = np.ascontiguousarray(np.stack([np.stack([self.f[f"E{c}_m{m}"] for m in range(1, M + 1)]) for c in self.comps]))
+            Bc = np.ascontiguousarray(np.stack([np.stack([self.f[f"B{c}_m{m}"] for m in range(1, M + 1)]) for c in self.comps]))
+        else:
+            Ec = Bc = np.zeros(2)
+        for parts in species:
+            if parts.immobile or parts.Args["Np"] == 0:
+                continue
+            d = parts.data
+            lib().orc_gather_push(_p(d["x"]), _p(d["y"]), _p(d["z"]), _p(d["px"]),
+                                  _p(d["py"]), _p(d["pz"]), _p(d["g_inv"]),
+                                  _p(d["sort_indx"]), parts.Args["Np"],
+                                  parts.Args["Np_stay"], parts.Args["FactorPush"],
+                                  a["Nx"], a["Xmin"], a["dx_inv"], a["Nr"], a["Rmin"],
+                                  a["dr_inv"], M, _p(E0), _p(B0), _p(Ec), _p(Bc))
+
+
+def add_gausian_pulse(solver, laser):
+    """laser.py:3-37."""
+    k0 = 2 * np.pi * laser["k0"]
+    a0, Lx, R, x0, x_foc = (laser[k] for k in ("a0", "Lx", "R", "x0", "x_foc"))
+    X_focus = x0 - x_foc
+    a = solver.Args
+    Xg, Rg = a["Xgrid"], a["Rgrid"]
+    kx, w = a["kx"][None, :], solver.sp["w"][0]
+    solver.f["Ez_m0"][1:] = (a0 * np.sin(k0 * (Xg[None, :] - x0))
+                             * np.exp(-(Xg[None, :] - x0) ** 2 / Lx ** 2 - Rg[1:, None] ** 2 / R ** 2)
+                             * (abs(Rg[1:, None]) < 3.5 * R) * (abs(Xg[None, :] - x0) < 3.5 * Lx))
+    solver.fb_transform(scals=["Ez"], dir=0)
+    EE = solver.f["Ez_fb_m0"].copy()
+    DT = -1j * w * np.sign(kx + (kx == 0))
+    GG = DT * EE
+    EE_tmp = np.cos(w * X_focus) * EE + np.sin(w * X_focus) / w * GG
+    GG = -w * np.sin(w * X_focus) * EE + np.cos(w * X_focus) * GG
+    EE = EE_tmp
+    EE = EE * np.exp(1j * kx * X_focus)
+    GG = GG * np.exp(1j * kx * X_focus)
+    solver.f["Ez_fb_m0"][:] = EE
+    solver.f["Gz_fb_m0"][:] = GG
+    solver.restore_B_fb()
+    solver.fb_transform(vects=["B", "E"], dir=1)
+
+
+class OracleLoop:
+    """PIC_loop.step, pic_loop.py:57-142 (no frames, no diagnostics).
+
+    ``record`` keeps the charge-conservation inputs of the last step."""
+
+    def __init__(self, solver, species):
+        self.solver, self.species, self.it = solver, species, 0
+
+    def step(self):
+        s = self.solver
+        for p in self.species:
+            p.push_coords("half")
+            p.sort_parts(s)
+        s.depose_currents(self.species)
+        for p in self.species:
+            p.push_coords("half")
+            p.sort_parts(s)
+        s.depose_charge(self.species)
+        s.fb_transform(scals=["rho"], vects=["J"], dir=0)
+        s.fields_smooth(["rho", "Jx", "Jy", "Jz"])
+        for m in range(s.M + 1):
+            for c in s.comps:
+                s.f[f"dN0{c}_fb_m{m}"][:] = s.f[f"dN1{c}_fb_m{m}"]
+        s.field_grad("rho", "dN1")
+        s.push_fields()
+        s.damp_fields()
+        s.restore_B_fb()
+        s.fb_transform(vects=["E", "B"], dir=1)
+        s.gather_and_push(self.species)
+        self.it += 1
+        return self.it

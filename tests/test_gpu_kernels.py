@@ -1,0 +1,223 @@
+"""GPU kernels vs the CPU oracle on identical inputs (through the C ABI, via
+the reference-API wrappers).  Integer work and the FP64 particle kernels are
+bit-exact; the transforms and spectral operators agree to <= 1e-12 relative
+L2 (summation order of the FFT / MFMA products differs)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+@pytest.fixture(scope="module")
+def env():
+    _require_gpu()
+    from tests import simpair as S
+    return S
+
+
+def _rand_particles(g, n, seed, inside_frac=0.97):
+    rng = np.random.default_rng(seed)
+    dx = (g["Xmax"] - g["Xmin"]) / (g["Nx"] - 1)
+    dr = g["Rmax"] / (g["Nr"] - 1.5)
+    x = rng.uniform(g["Xmin"] - 2 * dx, g["Xmax"] + 2 * dx, n)
+    r = rng.uniform(0.01 * dr, (g["Nr"] - 0.5) * dr, n)
+    th = rng.uniform(0, 2 * np.pi, n)
+    # some particles exactly on cell boundaries
+    k = n // 20
+    x[:k] = g["Xmin"] + dx * rng.integers(0, g["Nx"], k)
+    return {"x": x, "y": r * np.sin(th), "z": r * np.cos(th),
+            "px": rng.normal(0, 0.3, n), "py": rng.normal(0, 0.3, n), "pz": rng.normal(0, 0.3, n),
+            "w": rng.uniform(0.5, 1.5, n) * 1e-4, "g_inv": rng.uniform(0.5, 1.0, n)}
+
+
+def _pair_particles(env, g, n, seed, M):
+    import chimeracl_amd as CA
+    from chimeracl_amd import config as C
+    from chimeracl_amd.methods.generic_methods_hip import Communicator
+    from oracle import oracle as O
+    comm = Communicator(precision="f64")
+    solver = CA.Solver(dict(g), comm)
+    cfg = {"Nppc": (2, 2, 4), "dx": solver.Args["dx"], "dr": solver.Args["dr"], "dt": solver.Args["dt"],
+           "dens": 0.002, "charge": -1}
+    arr = _rand_particles(g, n, seed)
+    p = CA.Particles(dict(cfg), comm)
+    p.add_particles(source=C.ParticleSource(arr))
+    s = O.OracleSolver(dict(g))
+    q = O.OracleParticles(dict(cfg))
+    q.add_particles(arr)
+    return solver, p, s, q
+
+
+GRIDS = [dict(Nx=64, Nr=24, M=1), dict(Nx=128, Nr=40, M=0), dict(Nx=64, Nr=32, M=2)]
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_push_sort_deposit_bit_exact(env, gi):
+    g = env.tiny_grid(**GRIDS[gi])
+    solver, p, s, q = _pair_particles(env, g, 20000, 10 + gi, g["M"])
+    p.push_coords("half")
+    q.push_coords("half")
+    for k in ("x", "y", "z"):
+        assert np.array_equal(p.host(k), q.data[k]), k
+    p.sort_parts(solver)
+    q.sort_parts(s)
+    key = p.DataDev["indx_in_cell"].cpu().numpy().view(np.uint32)
+    assert np.array_equal(key, q.data["indx_in_cell"])
+    assert np.array_equal(p.DataDev["cell_offset"].cpu().numpy().view(np.uint32), q.data["cell_offset"])
+    srt = p.DataDev["sort_indx"].cpu().numpy().view(np.uint32)
+    assert np.array_equal(srt, np.argsort(key, kind="stable").astype(np.uint32))
+    assert np.array_equal(srt, q.data["sort_indx"])
+    assert p.Args["Np_stay"] == q.Args["Np_stay"]
+    assert p.sort_flags() == 0
+    # deposits: bit-exact, including the node colour order
+    solver.depose_charge([p])
+    s.depose_charge([q])
+    solver.depose_currents([p])
+    s.depose_currents([q])
+    for name in env.real_names(g["M"], flds=("J",)):
+        a = env.field_host(solver, name)
+        b = s.f[name]
+        assert np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_gather_push_bit_exact(env, gi):
+    g = env.tiny_grid(**GRIDS[gi])
+    solver, p, s, q = _pair_particles(env, g, 20000, 20 + gi, g["M"])
+    rng = np.random.default_rng(5 + gi)
+    for f in ("E", "B"):
+        for c in "xyz":
+            for m in range(g["M"] + 1):
+                name = f"{f}{c}_m{m}"
+                shp = s.f[name].shape
+                v = rng.normal(0, 0.5, shp) if m == 0 else rng.normal(0, 0.5, shp) + 1j * rng.normal(0, 0.5, shp)
+                s.f[name][:] = v
+                solver.DataDev[name].copy_(torch.from_numpy(np.ascontiguousarray(v)))
+    p.sort_parts(solver)
+    q.sort_parts(s)
+    solver.gather_and_push([p])
+    s.gather_and_push([q])
+    for k in ("px", "py", "pz", "g_inv"):
+        assert np.array_equal(p.host(k), q.data[k]), k
+    for name in env.real_names(g["M"], flds=("E", "B"), scal=()):
+        assert np.array_equal(env.field_host(solver, name), s.f[name]), name
+
+
+def test_dens_profile_bit_exact(env):
+    g = env.tiny_grid()
+    solver, p, s, q = _pair_particles(env, g, 5000, 3, 1)
+    xx = [g["Xmin"] - 1, g["Xmin"] + 0.3, g["Xmin"] + 0.5, g["Xmax"] + 1]
+    ff = [0.0, 0.5, 1.0, 1.2]
+    p.dens_profile(xx, ff)
+    q.dens_profile(xx, ff)
+    assert np.array_equal(p.host("w"), q.data["w"])
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_tables(env, gi):
+    import chimeracl_amd as CA
+    from chimeracl_amd.methods.generic_methods_hip import Communicator
+    from oracle import oracle as O
+    g = env.tiny_grid(**GRIDS[gi])
+    solver = CA.Solver(dict(g), Communicator(precision="f64"))
+    s = O.OracleSolver(dict(g))
+    D = solver.DataDev
+    M = g["M"]
+    assert np.array_equal(D["kx"].cpu().numpy(), s.Args["kx"])
+    for m in range(M + 2):
+        assert np.allclose(D[f"kr_m{m}"].cpu().numpy(), s.sp["kr"][m], rtol=1e-14, atol=0)
+    for m in range(M + 1):
+        for mine, ref in (("DHT_inv_m", "DHT_inv"), ("DHT_m", "DHT"), ("dDHT_plus_m", "dDp"),
+                          ("dDHT_minus_m", "dDm")):
+            a, b = D[mine + str(m)].cpu().numpy(), s.dht[ref][m]
+            assert env.rel_l2(a, b) < 1e-12, (mine, m, env.rel_l2(a, b))
+        c1, c2, c3 = s.coef[m]
+        for key, ref in (("w_m", s.sp["w"][m]), ("MxSlv_cos(wdt)_m", c1), ("MxSlv_sin(wdt)*w_m", c2),
+                         ("MxSlv_1/w**2_m", c3), ("SmoothingFilter_m", s.sp["Smooth"][m])):
+            a = D[key + str(m)].cpu().numpy()
+            assert env.rel_l2(a, ref) < 1e-14, (key, m, env.rel_l2(a, ref))
+    assert np.allclose(D["DampProfile"].cpu().numpy(), s.damp, rtol=1e-14, atol=1e-15)
+    assert np.allclose(D["phase_minus"].cpu().numpy(), s.phase(0), rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_spectral_ops(env, gi):
+    import chimeracl_amd as CA
+    from chimeracl_amd.methods.generic_methods_hip import Communicator
+    from oracle import oracle as O
+    g = env.tiny_grid(**GRIDS[gi])
+    M = g["M"]
+    solver = CA.Solver(dict(g), Communicator(precision="f64"))
+    s = O.OracleSolver(dict(g))
+    rng = np.random.default_rng(40 + gi)
+
+    def setf(name, cplx):
+        shp = s.f[name].shape
+        v = rng.normal(0, 1, shp) + (1j * rng.normal(0, 1, shp) if cplx else 0)
+        s.f[name][...] = v
+        solver.DataDev[name].copy_(torch.from_numpy(np.ascontiguousarray(v)))
+
+    # forward transforms (full and half)
+    for c in "xyz":
+        for m in range(M + 1):
+            setf(f"J{c}_m{m}", m > 0)
+    for m in range(M + 1):
+        setf(f"rho_m{m}", m > 0)
+    solver.fb_transform(scals=["rho"], vects=["J"], dir=0)
+    s.fb_transform(scals=["rho"], vects=["J"], dir=0)
+    for m in range(M + 1):
+        for n in ["rho"] + ["J" + c for c in "xyz"]:
+            nm = f"{n}_fb_m{m}"
+            assert env.rel_l2(env.field_host(solver, nm), s.f[nm]) < 1e-12, nm
+    # smoothing, grad with the dN0 copy
+    solver.fields_smooth(["rho", "Jx", "Jy", "Jz"])
+    s.fields_smooth(["rho", "Jx", "Jy", "Jz"])
+    for c in "xyz":
+        for m in range(M + 1):
+            setf(f"dN1{c}_fb_m{m}", True)
+    old = {f"dN1{c}_fb_m{m}": s.f[f"dN1{c}_fb_m{m}"].copy() for c in "xyz" for m in range(M + 1)}
+    solver.field_grad("rho", "dN1", copy_to="dN0")
+    s.field_grad("rho", "dN1")
+    for c in "xyz":
+        for m in range(M + 1):
+            nm = f"dN1{c}_fb_m{m}"
+            assert env.rel_l2(env.field_host(solver, nm), s.f[nm]) < 1e-12, nm
+            assert np.array_equal(env.field_host(solver, f"dN0{c}_fb_m{m}"), old[nm])
+            s.f[f"dN0{c}_fb_m{m}"][:] = old[nm]
+    # PSATD + damping fused vs the oracle's sequence
+    for f in ("E", "G"):
+        for c in "xyz":
+            for m in range(M + 1):
+                setf(f"{f}{c}_fb_m{m}", True)
+    solver.damp_edges(("E", "G"), advance_with=("J", "dN0", "dN1"))
+    s.push_fields()
+    s.damp_fields()
+    for f in ("E", "G"):
+        for c in "xyz":
+            for m in range(M + 1):
+                nm = f"{f}{c}_fb_m{m}"
+                assert env.rel_l2(env.field_host(solver, nm), s.f[nm]) < 1e-12, nm
+    # rot + Poisson, div, backward transforms
+    solver.restore_B_fb()
+    s.restore_B_fb()
+    for c in "xyz":
+        for m in range(M + 1):
+            nm = f"B{c}_fb_m{m}"
+            assert env.rel_l2(env.field_host(solver, nm), s.f[nm]) < 1e-12, nm
+    for m in range(M + 1):
+        solver.DataDev[f"divJ_fb_m{m}"] = torch.zeros_like(solver.DataDev[f"rho_fb_m{m}"])
+    solver.field_div("J", "divJ")
+    dref = s.field_div("J")
+    for m in range(M + 1):
+        assert env.rel_l2(env.field_host(solver, f"divJ_fb_m{m}"), dref[m]) < 1e-12
+    solver.fb_transform(vects=["E", "B"], dir=1)
+    s.fb_transform(vects=["E", "B"], dir=1)
+    for n in env.real_names(M, flds=("E", "B"), scal=()):
+        assert env.rel_l2(env.field_host(solver, n)[1:], s.f[n][1:]) < 1e-12, n
