@@ -5,10 +5,11 @@
 // Deposition (replaces the 4-colour depose_scalar / depose_vector launches,
 // grid_deposit_m1.cl:20-327, grid_methods_cl.py:44-95) runs in two passes and
 // uses no float atomics:
-//   pass 1 (one lane per valid cell): the cell's particles, in sorted order,
-//          are summed into private accumulators exactly as the reference
-//          kernel does; the 4 node corners x components are written to a
-//          cell-major staging buffer (coalesced, component-major).
+//   pass 1 (one lane per valid cell, particles staged through LDS): the
+//          cell's particles, in sorted order, are summed into private
+//          accumulators exactly as the reference kernel does; the 4 node
+//          corners x components are written to a staging buffer
+//          (component-major, coalesced).
 //   pass 2 (one lane per node): each node adds the partial sums of its (up
 //          to) four cells in the reference's colour order 0,1,2,3 -- the order
 //          in which the reference's serialised colour launches add them.
@@ -29,22 +30,46 @@ struct PlanePtrs {
 };
 
 // ------------------------------------------------------------------ pass 1
+// One wave per DEP_CELLS consecutive cells.  Their particles occupy one
+// contiguous range of sorted slots, which the wave stages into LDS in
+// chunks of DEP_CAP slots with coalesced loads (sort_indx is close to the
+// identity for a cell-major plasma, so the gathers are nearly coalesced too).
+// Lane l < DEP_CELLS then sums the particles of cell c0 + l sequentially, in
+// sorted order, with the reference kernel's arithmetic.
+#define DEP_CELLS 16
+#define DEP_CAP 512
+// Lanes walk their own cell's slots, ~ppc apart: one pad element per 32
+// spreads them over the LDS banks.
+#define DEP_SW(t) ((t) + ((t) >> 5))
+
 template <typename T, int M, bool VEC>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
     k_cell_sums(const uint32_t *__restrict__ sort_indx, const uint32_t *__restrict__ cell_offset,
                 const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
                 const T *__restrict__ ux, const T *__restrict__ uy, const T *__restrict__ uz,
                 const T *__restrict__ g_inv, const T *__restrict__ w, T charge, int Nx, int Nr,
                 T xmin, T dx_inv, T rmin, T dr_inv, T *__restrict__ part) {
   constexpr int NK = VEC ? 3 : 1;
+  constexpr int NV = VEC ? 8 : 4;
+  __shared__ T st[NV][DEP_CAP + DEP_CAP / 32];
   const int Nx_cell = Nx - 1, Nr_cell = Nr - 1;
   const int64_t ncell = (int64_t)Nx_cell * Nr_cell;
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int ir = (int)(c / Nx_cell), ix = (int)(c - (int64_t)ir * Nx_cell);
-  if (!(ix > 0 && ix < Nx_cell - 1 && ir < Nr_cell - 1)) return;
-  uint32_t ip0 = cell_offset[c], ip1 = cell_offset[c + 1];
-  if (ip0 == ip1) return;
+  const int lane = threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * DEP_CELLS;
+  const int64_t c1 = (c0 + DEP_CELLS < ncell) ? c0 + DEP_CELLS : ncell;
+  const uint32_t S0 = cell_offset[c0], S1 = cell_offset[c1];
+  const int64_t c = c0 + lane;
+  bool act = lane < DEP_CELLS && c < ncell;
+  int ir = 0, ix = 0;
+  uint32_t ip0 = 0, ip1 = 0;
+  if (act) {
+    ir = (int)(c / Nx_cell);
+    ix = (int)(c - (int64_t)ir * Nx_cell);
+    act = ix > 0 && ix < Nx_cell - 1 && ir < Nr_cell - 1;
+    ip0 = cell_offset[c];
+    ip1 = cell_offset[c + 1];
+    act = act && ip0 != ip1;
+  }
   T a0[NK][2][2];
   T ar[NK][M > 0 ? M : 1][2][2], ai[NK][M > 0 ? M : 1][2][2];
 #pragma unroll
@@ -57,71 +82,94 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int m = 0; m < (M > 0 ? M : 1); m++) ar[k][m][i][j] = ai[k][m][i][j] = 0;
       }
-  for (uint32_t ip = ip0; ip < ip1; ip++) {
-    uint32_t p = sort_indx[ip];
-    T xp = x[p], yp = y[p], zp = z[p];
-    T jp[3];
-    T wp;
-    if (VEC) {
-      jp[0] = ux[p];
-      jp[1] = uy[p];
-      jp[2] = uz[p];
-      wp = w[p] * g_inv[p] * charge;
-    } else {
-      wp = w[p] * charge;
+  for (uint32_t cs = S0; cs < S1; cs += DEP_CAP) {
+    const uint32_t n = (S1 - cs < DEP_CAP) ? S1 - cs : DEP_CAP;
+    for (uint32_t t = lane; t < n; t += 64) {
+      uint32_t p = sort_indx[cs + t];
+      const uint32_t q = DEP_SW(t);
+      st[0][q] = x[p];
+      st[1][q] = y[p];
+      st[2][q] = z[p];
+      st[3][q] = w[p];
+      if (VEC) {
+        st[4][q] = ux[p];
+        st[5][q] = uy[p];
+        st[6][q] = uz[p];
+        st[7][q] = g_inv[p];
+      }
     }
-    T rp = dsqrt<T>(yp * yp + zp * zp);
-    T rp_inv;
-    if (VEC) {
-      rp_inv = 0;
-      if (rp > 0) rp_inv = (T)1. / rp;
-    } else {
-      rp_inv = (T)1. / rp;
-    }
-    // ((y + i z)/r)^m, m = 1..M
-    T er[M > 0 ? M + 1 : 2], ei[M > 0 ? M + 1 : 2];
-    er[1] = yp * rp_inv;
-    ei[1] = zp * rp_inv;
-#pragma unroll
-    for (int m = 2; m <= M; m++) {
-      er[m] = er[m - 1] * er[1] - ei[m - 1] * ei[1];
-      ei[m] = er[m - 1] * ei[1] + ei[m - 1] * er[1];
-    }
-    T sX1 = (xp - xmin) * dx_inv - (T)ix;
-    T sX0 = (T)1.0 - sX1;
-    T sR1 = (rp - rmin) * dr_inv - (T)ir;
-    T sR0 = (T)1.0 - sR1;
-    sX0 *= wp;
-    sX1 *= wp;
-    T C[2][2];
-    C[0][0] = sR0 * sX0;
-    C[0][1] = sR0 * sX1;
-    C[1][0] = sR1 * sX0;
-    C[1][1] = sR1 * sX1;
-#pragma unroll
-    for (int k = 0; k < NK; k++)
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-          if (VEC) {
-            T jproj = C[i][j] * jp[k];
-            a0[k][i][j] += C[i][j] * jp[k];
-#pragma unroll
-            for (int m = 1; m <= M; m++) {
-              ar[k][m - 1][i][j] += jproj * er[m];
-              ai[k][m - 1][i][j] += jproj * ei[m];
-            }
-          } else {
-            a0[k][i][j] += C[i][j];
-#pragma unroll
-            for (int m = 1; m <= M; m++) {
-              ar[k][m - 1][i][j] += C[i][j] * er[m];
-              ai[k][m - 1][i][j] += C[i][j] * ei[m];
-            }
-          }
+    __syncthreads();
+    if (act) {
+      uint32_t lo = ip0 > cs ? ip0 : cs, hi = ip1 < cs + n ? ip1 : cs + n;
+      for (uint32_t ip = lo; ip < hi; ip++) {
+        const uint32_t t = DEP_SW(ip - cs);
+        T xp = st[0][t], yp = st[1][t], zp = st[2][t];
+        T jp[3];
+        T wp;
+        if (VEC) {
+          jp[0] = st[4][t];
+          jp[1] = st[5][t];
+          jp[2] = st[6][t];
+          wp = st[3][t] * st[7][t] * charge;
+        } else {
+          wp = st[3][t] * charge;
         }
+        T rp = dsqrt<T>(yp * yp + zp * zp);
+        T rp_inv;
+        if (VEC) {
+          rp_inv = 0;
+          if (rp > 0) rp_inv = (T)1. / rp;
+        } else {
+          rp_inv = (T)1. / rp;
+        }
+        // ((y + i z)/r)^m, m = 1..M
+        T er[M > 0 ? M + 1 : 2], ei[M > 0 ? M + 1 : 2];
+        er[1] = yp * rp_inv;
+        ei[1] = zp * rp_inv;
+#pragma unroll
+        for (int m = 2; m <= M; m++) {
+          er[m] = er[m - 1] * er[1] - ei[m - 1] * ei[1];
+          ei[m] = er[m - 1] * ei[1] + ei[m - 1] * er[1];
+        }
+        T sX1 = (xp - xmin) * dx_inv - (T)ix;
+        T sX0 = (T)1.0 - sX1;
+        T sR1 = (rp - rmin) * dr_inv - (T)ir;
+        T sR0 = (T)1.0 - sR1;
+        sX0 *= wp;
+        sX1 *= wp;
+        T C[2][2];
+        C[0][0] = sR0 * sX0;
+        C[0][1] = sR0 * sX1;
+        C[1][0] = sR1 * sX0;
+        C[1][1] = sR1 * sX1;
+#pragma unroll
+        for (int k = 0; k < NK; k++)
+#pragma unroll
+          for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+              if (VEC) {
+                T jproj = C[i][j] * jp[k];
+                a0[k][i][j] += C[i][j] * jp[k];
+#pragma unroll
+                for (int m = 1; m <= M; m++) {
+                  ar[k][m - 1][i][j] += jproj * er[m];
+                  ai[k][m - 1][i][j] += jproj * ei[m];
+                }
+              } else {
+                a0[k][i][j] += C[i][j];
+#pragma unroll
+                for (int m = 1; m <= M; m++) {
+                  ar[k][m - 1][i][j] += C[i][j] * er[m];
+                  ai[k][m - 1][i][j] += C[i][j] * ei[m];
+                }
+              }
+            }
+      }
+    }
+    __syncthreads();
   }
+  if (!act) return;
   // staging layout: part[((k*(1+2M) + q)*4 + corner) * ncell + c]
   constexpr int NQ = 1 + 2 * M;
 #pragma unroll
@@ -201,9 +249,10 @@ template <typename T, int M, bool VEC>
 static void launch_deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const T *x,
                            const T *y, const T *z, const T *ux, const T *uy, const T *uz,
                            const T *g_inv, const T *w, T charge, const chm_grid_t *g,
-                           const PlanePtrs &pp, T *part, hipStream_t st) {
+                           const PlanePtrs &pp, T *part, bool pass1, hipStream_t st) {
   int64_t ncell = (int64_t)(g->Nx - 1) * (g->Nr - 1);
-  k_cell_sums<T, M, VEC><<<chm_blocks(ncell, 256), 256, 0, st>>>(
+  if (pass1)
+    k_cell_sums<T, M, VEC><<<chm_blocks(ncell, DEP_CELLS), 64, 0, st>>>(
       sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr, (T)g->xmin,
       (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, part);
   int64_t nnode = (int64_t)(g->Nr - 1) * g->Nx;
@@ -215,7 +264,7 @@ template <typename T, bool VEC>
 static int deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const T *x, const T *y,
                    const T *z, const T *ux, const T *uy, const T *uz, const T *g_inv, const T *w,
                    int32_t charge, const chm_grid_t *g, const chm_planes_t *flds, void *ws,
-                   size_t ws_bytes, void *stream) {
+                   size_t ws_bytes, void *stream, bool pass1 = true) {
   if (!g || !flds || g->M < 0 || g->M > CHM_MAXM || g->Nx < 4 || g->Nr < 3) return CHM_EINVAL;
   if (chm_deposit_workspace_bytes(g->Nx, g->Nr, g->M, VEC) > ws_bytes || !ws) return CHM_EWORKSPACE;
   PlanePtrs pp = {};
@@ -231,7 +280,7 @@ static int deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const
 #define CASE(MM)                                                                                   \
   case MM:                                                                                         \
     launch_deposit<T, MM, VEC>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, q, g, pp,    \
-                               part, st);                                                          \
+                               part, pass1, st);                                                   \
     break;
     CASE(0) CASE(1) CASE(2) CASE(3) CASE(4)
 #undef CASE
@@ -273,6 +322,27 @@ extern "C" int chm_deposit_vector_f32(const uint32_t *sort_indx, const uint32_t 
                                       size_t ws_bytes, void *stream) {
   return deposit<float, true>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g, J,
                               ws, ws_bytes, stream);
+}
+
+// Second pass alone: adds the per-cell sums a previous deposit call left in
+// ws (for a species whose particles have not changed since).
+extern "C" int chm_deposit_add_staged_f64(const uint32_t *cell_offset, const chm_grid_t *g,
+                                          const chm_planes_t *flds, int32_t vector, void *ws,
+                                          size_t ws_bytes, void *stream) {
+  if (vector)
+    return deposit<double, true>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                 nullptr, nullptr, nullptr, 0, g, flds, ws, ws_bytes, stream, false);
+  return deposit<double, false>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                nullptr, nullptr, nullptr, 0, g, flds, ws, ws_bytes, stream, false);
+}
+extern "C" int chm_deposit_add_staged_f32(const uint32_t *cell_offset, const chm_grid_t *g,
+                                          const chm_planes_t *flds, int32_t vector, void *ws,
+                                          size_t ws_bytes, void *stream) {
+  if (vector)
+    return deposit<float, true>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                nullptr, nullptr, nullptr, 0, g, flds, ws, ws_bytes, stream, false);
+  return deposit<float, false>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               nullptr, nullptr, nullptr, 0, g, flds, ws, ws_bytes, stream, false);
 }
 
 // ------------------------------------------------------------------ post-processing

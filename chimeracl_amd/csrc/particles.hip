@@ -3,27 +3,25 @@
 //
 // Sort design (replaces index_and_sum_in_cell + PyOpenCL cumsum + the
 // atomic `sort` kernel, particles_generic.cl:88-126, 186-200):
-//   1. zero the per-bin counters
-//   2. key kernel: cell key per particle, run-aggregated atomic histogram
-//      (one atomic per run of equal keys inside a wave), and the number of
-//      out-of-domain particles per 256-particle block
-//   3. one exclusive scan over [histogram | per-block out-of-domain counts]
-//   4. scatter: valid keys take a slot from an atomic per-bin cursor
-//      (run-aggregated, so each run lands contiguously in index order);
-//      out-of-domain particles take a deterministic slot from the block scan
-//   5. in-cell fix-up: a bin whose slots are not in ascending particle order
-//      (runs from different waves raced) is sorted in place -- in registers
-//      through global memory for <= 64 particles, in LDS by one workgroup
-//      (bitonic) for up to 16384.
-// The result is exactly numpy.argsort(key, kind='stable').
+//   1. key kernel: cell key per particle (optionally fused with the
+//      coordinate push) and a run-aggregated atomic histogram (one atomic per
+//      run of equal keys inside a wave, no return value used)
+//   2. exclusive scan of the histogram -> cell_offset, Np_stay
+//   3. stable LSD radix sort of the keys with the particle index as value,
+//      8-bit digits: per 4096-key tile an LDS digit histogram, one scan over
+//      [digit][tile], then a scatter that ranks keys inside the tile in index
+//      order (ballot-matched digits per wave, tagged per-wave counts in LDS).
+// The result is exactly numpy.argsort(key, kind='stable'); no float or
+// returning global atomics, no order fix-up.
 #include "chm_common.h"
 
 #define SORT_BS 256
 #define SCAN_ITEMS 8
 #define SCAN_BS 256
 #define SCAN_TILE (SCAN_BS * SCAN_ITEMS)
-#define FIX_SMALL 64
-#define FIX_BIG 16384
+#define RS_BS 256
+#define RS_ITEMS 16
+#define RS_TILE (RS_BS * RS_ITEMS)
 
 // ------------------------------------------------------------------ push
 template <typename T>
@@ -96,22 +94,37 @@ __device__ __forceinline__ unsigned long long wave_runs(uint32_t rkey, int lane,
   return heads;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(SORT_BS) k_cell_key(const T *__restrict__ x, const T *__restrict__ y,
-                                                      const T *__restrict__ z, int64_t Np, int Nx, int Nr,
-                                                      T xmin, T dx_inv, T rmin, T dr_inv,
+// With PUSH, the coordinate push of push_xyz (same rounding order) is applied
+// first and the key is taken from the pushed position.
+template <typename T, bool PUSH>
+__global__ void __launch_bounds__(SORT_BS) k_cell_key(T *__restrict__ x, T *__restrict__ y,
+                                                      T *__restrict__ z, const T *__restrict__ px,
+                                                      const T *__restrict__ py, const T *__restrict__ pz,
+                                                      const T *__restrict__ g_inv, T dt, int64_t Np,
+                                                      int Nx, int Nr, T xmin, T dx_inv, T rmin, T dr_inv,
                                                       uint32_t *__restrict__ key_out,
-                                                      uint32_t *__restrict__ hist,
-                                                      uint32_t *__restrict__ ovf_cnt) {
-  __shared__ uint32_t wcnt[SORT_BS / CHM_WAVE];
+                                                      uint32_t *__restrict__ hist) {
   const int Nx_loc = Nx - 1, Nr_loc = Nr - 1;
   const uint32_t ovf = (uint32_t)(Nr_loc * Nx_loc);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   int64_t ip = (int64_t)blockIdx.x * SORT_BS + threadIdx.x;
   bool act = ip < Np;
   uint32_t key = ovf;
   if (act) {
-    key = cell_key<T>(x[ip], y[ip], z[ip], Nx_loc, Nr_loc, xmin, dx_inv, rmin, dr_inv);
+    T xp = x[ip], yp = y[ip], zp = z[ip];
+    if (PUSH) {
+      T dt_g = dt * g_inv[ip];
+      T dx = px[ip] * dt_g;
+      T dy = py[ip] * dt_g;
+      T dz = pz[ip] * dt_g;
+      xp = xp + dx;
+      yp = yp + dy;
+      zp = zp + dz;
+      x[ip] = xp;
+      y[ip] = yp;
+      z[ip] = zp;
+    }
+    key = cell_key<T>(xp, yp, zp, Nx_loc, Nr_loc, xmin, dx_inv, rmin, dr_inv);
     key_out[ip] = key;
   }
   bool valid = act && key != ovf;
@@ -119,14 +132,6 @@ __global__ void __launch_bounds__(SORT_BS) k_cell_key(const T *__restrict__ x, c
   int len, hl;
   wave_runs(rkey, lane, &len, &hl);
   if (valid && hl == lane) atomicAdd(&hist[key], (uint32_t)len);
-  unsigned long long ob = __ballot(act && !valid);
-  if (lane == 0) wcnt[wave] = (uint32_t)__popcll((long long)ob);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t s = 0;
-    for (int w = 0; w < SORT_BS / CHM_WAVE; w++) s += wcnt[w];
-    ovf_cnt[blockIdx.x] = s;
-  }
 }
 
 __global__ void k_zero_u32(uint32_t *__restrict__ a, int64_t n) {
@@ -208,143 +213,126 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_final(uint32_t *__restrict__ i
   }
 }
 
-// ------------------------------------------------------------------ scatter
-__global__ void __launch_bounds__(SORT_BS) k_scatter(const uint32_t *__restrict__ key, int64_t Np,
-                                                     uint32_t nbins, uint32_t *__restrict__ cursor,
-                                                     const uint32_t *__restrict__ pre,
-                                                     uint32_t *__restrict__ sort_indx) {
-  __shared__ uint32_t wcnt[SORT_BS / CHM_WAVE];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t ovf = nbins - 1;
-  int64_t ip = (int64_t)blockIdx.x * SORT_BS + threadIdx.x;
-  bool act = ip < Np;
-  uint32_t k = act ? key[ip] : ovf;
-  bool valid = act && k != ovf;
-  uint32_t rkey = valid ? k : (0xFFFFFFFFu - (uint32_t)lane);
-  int len, hl;
-  wave_runs(rkey, lane, &len, &hl);
-  uint32_t base = 0;
-  if (valid && hl == lane) base = atomicAdd(&cursor[k], (uint32_t)len);
-  base = (uint32_t)__shfl((int)base, hl);
-  unsigned long long ob = __ballot(act && !valid);
-  if (lane == 0) wcnt[wave] = (uint32_t)__popcll((long long)ob);
-  __syncthreads();
-  if (valid) {
-    sort_indx[pre[k] + base + (uint32_t)(lane - hl)] = (uint32_t)ip;
-  } else if (act) {
-    uint32_t r = (uint32_t)__popcll((long long)(ob & ((1ull << lane) - 1)));
-    for (int w = 0; w < wave; w++) r += wcnt[w];
-    uint32_t slot = pre[ovf] + (pre[nbins + blockIdx.x] - pre[nbins]) + r;
-    sort_indx[slot] = (uint32_t)ip;
-  }
-}
-
-// ------------------------------------------------------------------ fix-up
-__global__ void __launch_bounds__(256) k_fixup(const uint32_t *__restrict__ pre, uint32_t nbins,
-                                               uint32_t Np, uint32_t *__restrict__ sort_indx,
-                                               uint32_t *__restrict__ big_list,
-                                               uint32_t *__restrict__ big_count,
-                                               uint32_t *__restrict__ cell_offset,
-                                               uint32_t *__restrict__ np_stay) {
+// ------------------------------------------------------------------ offsets
+// cell_offset[c] = exclusive scan of the histogram; the out-of-domain bin is
+// not counted in the histogram, so its start is Np_stay and the end is Np.
+__global__ void k_offsets(const uint32_t *__restrict__ pre, uint32_t nbins, uint32_t Np,
+                          uint32_t *__restrict__ cell_offset, uint32_t *__restrict__ np_stay) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  // the out-of-domain bin is counted per block, after the histogram, so the
-  // scan's entry at nbins is Np_stay; the last offset is Np
   if (c < (int64_t)nbins) cell_offset[c] = pre[c];
   if (c == (int64_t)nbins) cell_offset[c] = Np;
   if (c == 0) *np_stay = pre[nbins - 1];
-  if (c >= (int64_t)nbins - 1) return;
-  uint32_t s = pre[c], e = pre[c + 1];
-  if (e - s < 2) return;
-  bool sorted = true;
-  uint32_t prev = sort_indx[s];
-  for (uint32_t i = s + 1; i < e; i++) {
-    uint32_t v = sort_indx[i];
-    if (v < prev) { sorted = false; break; }
-    prev = v;
-  }
-  if (sorted) return;
-  if (e - s <= FIX_SMALL) {
-    for (uint32_t i = s + 1; i < e; i++) {
-      uint32_t v = sort_indx[i];
-      uint32_t j = i;
-      while (j > s && sort_indx[j - 1] > v) {
-        sort_indx[j] = sort_indx[j - 1];
-        j--;
-      }
-      sort_indx[j] = v;
-    }
-  } else {
-    uint32_t slot = atomicAdd(big_count, 1u);
-    big_list[slot] = (uint32_t)c;
-  }
 }
 
-__global__ void __launch_bounds__(1024) k_bigfix(const uint32_t *__restrict__ pre,
-                                                 uint32_t *__restrict__ sort_indx,
-                                                 const uint32_t *__restrict__ big_list,
-                                                 const uint32_t *__restrict__ big_count,
-                                                 uint32_t *__restrict__ flags) {
-  __shared__ uint32_t buf[FIX_BIG];
-  const uint32_t nbig = *big_count;
-  for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
-    uint32_t c = big_list[b];
-    uint32_t s = pre[c], n = pre[c + 1] - s;
-    if (n > FIX_BIG) {
-      if (threadIdx.x == 0) atomicOr(flags, 1u);
-      continue;
+// ------------------------------------------------------------------ radix sort
+__global__ void __launch_bounds__(RS_BS) k_rs_upsweep(const uint32_t *__restrict__ keys, int64_t n,
+                                                      int shift, uint32_t mask,
+                                                      uint32_t *__restrict__ counts, int64_t nblk) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+  for (int i = 0; i < RS_ITEMS; i++) {
+    int64_t idx = base + (int64_t)i * RS_BS + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  counts[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of one tile.  Items are visited in index order: round r,
+// wave w, lane l holds item base + r*256 + 64*w + l.  Within a wave, lanes
+// with equal digits are matched by ballots; per-wave digit counts go to LDS
+// tagged with the round number (no clearing needed); run[d] is the tile's
+// running output position of digit d.
+__global__ void __launch_bounds__(RS_BS) k_rs_downsweep(const uint32_t *__restrict__ kin,
+                                                        const uint32_t *__restrict__ vin, int64_t n,
+                                                        int shift, int nbits,
+                                                        const uint32_t *__restrict__ offs, int64_t nblk,
+                                                        uint32_t *__restrict__ kout,
+                                                        uint32_t *__restrict__ vout) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wc[RS_BS / CHM_WAVE][256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t mask = (1u << nbits) - 1u;
+  run[tid] = offs[(int64_t)tid * nblk + blockIdx.x];
+  for (int w = 0; w < RS_BS / CHM_WAVE; w++) wc[w][tid] = 0;
+  __syncthreads();
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ITEMS; r++) {
+    const int64_t idx = base + (int64_t)r * RS_BS + tid;
+    const bool act = idx < n;
+    uint32_t k = act ? kin[idx] : 0u;
+    uint32_t v = act ? (vin ? vin[idx] : (uint32_t)idx) : 0u;
+    uint32_t d = (k >> shift) & mask;
+    unsigned long long peers = __ballot(act);
+    for (int b = 0; b < nbits; b++) {
+      bool bit = (d >> b) & 1u;
+      unsigned long long bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
     }
-    uint32_t P = 1;
-    while (P < n) P <<= 1;
-    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) buf[t] = t < n ? sort_indx[s + t] : 0xFFFFFFFFu;
+    const uint32_t rank = (uint32_t)__popcll((long long)(peers & lt));
+    const uint32_t cnt = (uint32_t)__popcll((long long)peers);
+    const bool leader = act && rank == 0;
+    const uint32_t tag = (uint32_t)(r + 1) << 16;
+    if (leader) wc[wave][d] = tag | cnt;
     __syncthreads();
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
-          uint32_t q = t ^ j;
-          if (q > t) {
-            uint32_t a = buf[t], bq = buf[q];
-            bool up = (t & k) == 0;
-            if ((a > bq) == up) {
-              buf[t] = bq;
-              buf[q] = a;
-            }
-          }
-        }
-        __syncthreads();
+    uint32_t dest = 0;
+    if (act) {
+      dest = run[d] + rank;
+      for (int w = 0; w < wave; w++) {
+        uint32_t e = wc[w][d];
+        if ((e & 0xFFFF0000u) == tag) dest += e & 0xFFFFu;
       }
     }
-    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) sort_indx[s + t] = buf[t];
     __syncthreads();
+    if (leader) atomicAdd(&run[d], cnt);
+    if (act) {
+      if (kout) kout[dest] = k;
+      vout[dest] = v;
+    }
   }
 }
 
 // ------------------------------------------------------------------ driver
 struct SortWs {
-  uint32_t *cnt, *pre, *bsum, *big_count, *flags, *big_list;
-  int64_t nblk, ncnt, nscan;
+  uint32_t *hist, *pre, *bsum, *counts, *offs, *bsum2, *kA, *kB, *vA, *vB;
+  int64_t nscan, nblk, ncounts, nscan2;
 };
 
 static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
-  int64_t nblk = (Np + SORT_BS - 1) / SORT_BS;
-  int64_t ncnt = nbins + nblk;
-  int64_t nscan = (ncnt + SCAN_TILE - 1) / SCAN_TILE;
+  int64_t nblk = (Np + RS_TILE - 1) / RS_TILE;
+  if (nblk < 1) nblk = 1;
+  int64_t ncounts = 256 * nblk;
+  int64_t nscan = (nbins + SCAN_TILE - 1) / SCAN_TILE;
+  int64_t nscan2 = (ncounts + SCAN_TILE - 1) / SCAN_TILE;
   size_t off = 0;
-  char *b = (char *)base;
-  size_t o_cnt = off; off += chm_align(ncnt * 4);
-  size_t o_pre = off; off += chm_align((ncnt + 1) * 4);
+  size_t o_hist = off; off += chm_align(nbins * 4);
+  size_t o_pre = off; off += chm_align((nbins + 1) * 4);
   size_t o_bsum = off; off += chm_align(nscan * 4);
-  size_t o_misc = off; off += 256;  // big_count, flags
-  size_t o_big = off; off += chm_align(nbins * 4);
+  size_t o_counts = off; off += chm_align(ncounts * 4);
+  size_t o_offs = off; off += chm_align((ncounts + 1) * 4);
+  size_t o_bsum2 = off; off += chm_align(nscan2 * 4);
+  size_t o_k = off; off += 2 * chm_align(Np * 4);
+  size_t o_v = off; off += 2 * chm_align(Np * 4);
   if (w) {
-    w->cnt = (uint32_t *)(b + o_cnt);
+    char *b = (char *)base;
+    w->hist = (uint32_t *)(b + o_hist);
     w->pre = (uint32_t *)(b + o_pre);
     w->bsum = (uint32_t *)(b + o_bsum);
-    w->big_count = (uint32_t *)(b + o_misc);
-    w->flags = (uint32_t *)(b + o_misc + 4);
-    w->big_list = (uint32_t *)(b + o_big);
+    w->counts = (uint32_t *)(b + o_counts);
+    w->offs = (uint32_t *)(b + o_offs);
+    w->bsum2 = (uint32_t *)(b + o_bsum2);
+    w->kA = (uint32_t *)(b + o_k);
+    w->kB = (uint32_t *)(b + o_k + chm_align(Np * 4));
+    w->vA = (uint32_t *)(b + o_v);
+    w->vB = (uint32_t *)(b + o_v + chm_align(Np * 4));
     w->nblk = nblk;
-    w->ncnt = ncnt;
+    w->ncounts = ncounts;
     w->nscan = nscan;
+    w->nscan2 = nscan2;
   }
   return off;
 }
@@ -353,33 +341,76 @@ extern "C" size_t chm_sort_workspace_bytes(int64_t Np, int64_t nbins) {
   return sort_layout(Np, nbins, nullptr, nullptr);
 }
 
-template <typename T>
-static int index_sort(const T *x, const T *y, const T *z, int64_t Np, const chm_grid_t *g,
-                      uint32_t *indx_in_cell, uint32_t *cell_offset, uint32_t *sort_indx,
-                      uint32_t *np_stay, void *ws, size_t ws_bytes, void *stream) {
-  if (!g || Np < 0 || Np > 0xFFFFFFF0ll || g->Nx < 4 || g->Nr < 3) return CHM_EINVAL;
+// Stage 1: zero the histogram, then keys + histogram (+ the push with PUSH).
+template <typename T, bool PUSH>
+static int sort_keys(T *x, T *y, T *z, const T *px, const T *py, const T *pz, const T *g_inv, T dt,
+                     int64_t Np, const chm_grid_t *g, uint32_t *indx_in_cell, void *ws, size_t ws_bytes,
+                     void *stream) {
+  if (!g || Np < 0 || Np > 0x7FFFFFF0ll || g->Nx < 4 || g->Nr < 3) return CHM_EINVAL;
   int64_t nbins = (int64_t)(g->Nx - 1) * (g->Nr - 1) + 1;
   SortWs w;
   if (sort_layout(Np, nbins, &w, ws) > ws_bytes || !ws) return CHM_EWORKSPACE;
   hipStream_t st = (hipStream_t)stream;
-  // zero the counters and the big-bin count (flags stay sticky)
-  k_zero_u32<<<chm_blocks(nbins, 256), 256, 0, st>>>(w.cnt, nbins);
-  k_zero_u32<<<1, 64, 0, st>>>(w.big_count, 1);
+  k_zero_u32<<<chm_blocks(nbins, 256), 256, 0, st>>>(w.hist, nbins);
   if (Np > 0)
-    k_cell_key<T><<<(unsigned)w.nblk, SORT_BS, 0, st>>>(x, y, z, Np, g->Nx, g->Nr, (T)g->xmin,
-                                                        (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv,
-                                                        indx_in_cell, w.cnt, w.cnt + nbins);
-  k_scan_reduce<<<(unsigned)w.nscan, SCAN_BS, 0, st>>>(w.cnt, w.ncnt, w.bsum);
-  k_scan_blocks<<<1, SCAN_BS, 0, st>>>(w.bsum, w.nscan);
-  k_scan_final<<<(unsigned)w.nscan, SCAN_BS, 0, st>>>(w.cnt, w.ncnt, nbins, w.bsum, w.pre);
-  if (Np > 0)
-    k_scatter<<<(unsigned)w.nblk, SORT_BS, 0, st>>>(indx_in_cell, Np, (uint32_t)nbins, w.cnt, w.pre,
-                                                    sort_indx);
-  k_fixup<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(w.pre, (uint32_t)nbins, (uint32_t)Np, sort_indx, w.big_list,
-                                                      w.big_count, cell_offset, np_stay);
-  k_bigfix<<<256, 1024, 0, st>>>(w.pre, sort_indx, w.big_list, w.big_count, w.flags);
+    k_cell_key<T, PUSH><<<chm_blocks(Np, SORT_BS), SORT_BS, 0, st>>>(
+        x, y, z, px, py, pz, g_inv, dt, Np, g->Nx, g->Nr, (T)g->xmin, (T)g->dx_inv, (T)g->rmin,
+        (T)g->dr_inv, indx_in_cell, w.hist);
   CHM_CHECK_LAUNCH();
   return CHM_OK;
+}
+
+static void scan_u32(const uint32_t *in, int64_t n, uint32_t *out, uint32_t *bsum, int64_t nscan,
+                     hipStream_t st) {
+  k_scan_reduce<<<(unsigned)nscan, SCAN_BS, 0, st>>>(in, n, bsum);
+  k_scan_blocks<<<1, SCAN_BS, 0, st>>>(bsum, nscan);
+  k_scan_final<<<(unsigned)nscan, SCAN_BS, 0, st>>>((uint32_t *)in, n, 0, bsum, out);
+}
+
+// Stage 2: offsets and the stable radix sort, from the keys of stage 1.
+static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_in_cell,
+                          uint32_t *cell_offset, uint32_t *sort_indx, uint32_t *np_stay, void *ws,
+                          size_t ws_bytes, void *stream) {
+  if (!g || Np < 0 || Np > 0x7FFFFFF0ll || g->Nx < 4 || g->Nr < 3) return CHM_EINVAL;
+  int64_t nbins = (int64_t)(g->Nx - 1) * (g->Nr - 1) + 1;
+  SortWs w;
+  if (sort_layout(Np, nbins, &w, ws) > ws_bytes || !ws) return CHM_EWORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  scan_u32(w.hist, nbins, w.pre, w.bsum, w.nscan, st);
+  k_offsets<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(w.pre, (uint32_t)nbins, (uint32_t)Np,
+                                                        cell_offset, np_stay);
+  if (Np == 0) {
+    CHM_CHECK_LAUNCH();
+    return CHM_OK;
+  }
+  int bits = 0;
+  while (((int64_t)1 << bits) < nbins) bits++;  // keys are < nbins
+  if (bits < 1) bits = 1;
+  const int npass = (bits + 7) / 8;
+  const uint32_t *kin = indx_in_cell, *vin = nullptr;
+  for (int p = 0; p < npass; p++) {
+    const int shift = 8 * p, nb = (bits - shift < 8) ? bits - shift : 8;
+    const bool last = p == npass - 1;
+    uint32_t *kout = last ? nullptr : ((p & 1) ? w.kB : w.kA);
+    uint32_t *vout = last ? sort_indx : ((p & 1) ? w.vB : w.vA);
+    k_rs_upsweep<<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, (1u << nb) - 1u, w.counts, w.nblk);
+    scan_u32(w.counts, w.ncounts, w.offs, w.bsum2, w.nscan2, st);
+    k_rs_downsweep<<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs, w.nblk, kout, vout);
+    kin = kout;
+    vin = vout;
+  }
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+template <typename T>
+static int index_sort(const T *x, const T *y, const T *z, int64_t Np, const chm_grid_t *g,
+                      uint32_t *indx_in_cell, uint32_t *cell_offset, uint32_t *sort_indx,
+                      uint32_t *np_stay, void *ws, size_t ws_bytes, void *stream) {
+  int rc = sort_keys<T, false>((T *)x, (T *)y, (T *)z, nullptr, nullptr, nullptr, nullptr, (T)0, Np, g,
+                               indx_in_cell, ws, ws_bytes, stream);
+  if (rc) return rc;
+  return sort_from_keys(Np, g, indx_in_cell, cell_offset, sort_indx, np_stay, ws, ws_bytes, stream);
 }
 
 extern "C" int chm_index_sort_f64(const double *x, const double *y, const double *z, int64_t Np,
@@ -397,11 +428,24 @@ extern "C" int chm_index_sort_f32(const float *x, const float *y, const float *z
                            ws_bytes, stream);
 }
 
-extern "C" size_t chm_sort_flags_offset(int64_t Np, int64_t nbins) {
-  char *base = (char *)(uintptr_t)4096;  // any aligned base; only the offset is used
-  SortWs w;
-  sort_layout(Np, nbins, &w, base);
-  return (size_t)((char *)w.flags - base);
+extern "C" int chm_push_x_keys_f64(double *x, double *y, double *z, const double *px, const double *py,
+                                   const double *pz, const double *g_inv, double dt, int64_t Np,
+                                   const chm_grid_t *g, uint32_t *indx_in_cell, void *ws,
+                                   size_t ws_bytes, void *stream) {
+  return sort_keys<double, true>(x, y, z, px, py, pz, g_inv, dt, Np, g, indx_in_cell, ws, ws_bytes,
+                                 stream);
+}
+extern "C" int chm_push_x_keys_f32(float *x, float *y, float *z, const float *px, const float *py,
+                                   const float *pz, const float *g_inv, float dt, int64_t Np,
+                                   const chm_grid_t *g, uint32_t *indx_in_cell, void *ws,
+                                   size_t ws_bytes, void *stream) {
+  return sort_keys<float, true>(x, y, z, px, py, pz, g_inv, dt, Np, g, indx_in_cell, ws, ws_bytes,
+                                stream);
+}
+extern "C" int chm_sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_in_cell,
+                                  uint32_t *cell_offset, uint32_t *sort_indx, uint32_t *np_stay,
+                                  void *ws, size_t ws_bytes, void *stream) {
+  return sort_from_keys(Np, g, indx_in_cell, cell_offset, sort_indx, np_stay, ws, ws_bytes, stream);
 }
 
 // ------------------------------------------------------------------ profile

@@ -53,13 +53,33 @@ class GridMethodsHIP(GenericMethodsHIP):
 
     # ---------------------------------------------------------------- deposits
     def depose_scalar(self, parts, src_scalar, dest_fld, charge):
-        """grid_methods_cl.py:44-69 (adds into '<dest_fld>_m*')."""
+        """grid_methods_cl.py:44-69 (adds into '<dest_fld>_m*').
+
+        An immobile species keeps its per-cell sums between calls: while its
+        particles are unchanged (Particles.version) only the node pass runs,
+        which adds the same sums in the same order."""
         if parts.Args["Np"] <= 0:
             return
-        d, ws = parts.DataDev, self._dep_ws()
+        d = parts.DataDev
+        planes = self._planes_arr([dest_fld])
+        if "Immobile" in parts.Args:
+            key = (id(parts), src_scalar, int(charge))
+            cache = self._pcache.setdefault("staged", {})
+            ent = cache.get(key)
+            if ent is not None and ent[0] == parts.version:
+                ws = ent[1]
+                self.call("chm_deposit_add_staged", ptr(d["cell_offset"]), self._grid(), planes, 0,
+                          ptr(ws), ws.numel())
+                return
+            a = self.Args
+            n = _lib.lib.chm_deposit_workspace_bytes(a["Nx"], a["Nr"], a["M"], 0)
+            ws = ent[1] if ent is not None else torch.empty(n, dtype=torch.uint8, device=self.comm.device)
+            cache[key] = (parts.version, ws)
+        else:
+            ws = self._dep_ws()
         self.call("chm_deposit_scalar", ptr(d["sort_indx"]), ptr(d["cell_offset"]), ptr(d["x"]),
                   ptr(d["y"]), ptr(d["z"]), ptr(d[src_scalar]), int(charge), self._grid(),
-                  self._planes_arr([dest_fld]), ptr(ws), ws.numel())
+                  planes, ptr(ws), ws.numel())
 
     def depose_vector(self, parts, vec, factors, vec_fld, charge):
         """grid_methods_cl.py:71-95 (adds w*g_inv*charge*u into '<vec_fld><c>_m*')."""

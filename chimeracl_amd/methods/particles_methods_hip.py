@@ -25,7 +25,9 @@ class ParticleMethodsHIP(GenericMethodsHIP):
     def init_particle_methods(self):
         self.set_global_working_group_size()
         self._sort_ws = None
-        self._np_stay_dev = None
+        self._keys_for = None
+        # bumped whenever positions, weights, momenta or the order change
+        self.version = 0
 
     def _arg_names(self):
         if "Immobile" in self.Args:
@@ -46,6 +48,7 @@ class ParticleMethodsHIP(GenericMethodsHIP):
                 else torch.cat([old, new])
         self.reset_num_parts()
         self.flag_sorted = False
+        self.version += 1
 
     def make_new_domain(self, parts_in, density_profiles=None):
         raise NotImplementedError(
@@ -76,17 +79,28 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         x, w = self.DataDev[coord], self.DataDev[weight]
         self.call("chm_dens_profile", ptr(x), ptr(w), x.numel(), *(ptr(t) for t in self._prof),
                   xx.size)
+        self.version += 1
 
     # ---------------------------------------------------------------- push
-    def push_coords(self, mode="half"):
-        """particles_methods_cl.py:206-223."""
+    def push_coords(self, mode="half", grid=None):
+        """particles_methods_cl.py:206-223.  With grid, the cell keys of the
+        next sort_parts(grid) are computed in the same pass."""
         if self.Args["Np"] == 0 or "Immobile" in self.Args:
             return
         dt = self.Args["dt_2"] if mode == "half" else self.Args["dt"]
         d = self.DataDev
-        self.call("chm_push_x", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]),
-                  ptr(d["py"]), ptr(d["pz"]), ptr(d["g_inv"]), dt, self.Args["Np"])
+        args = (ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]), ptr(d["py"]), ptr(d["pz"]),
+                ptr(d["g_inv"]), dt, self.Args["Np"])
+        if grid is None:
+            self.call("chm_push_x", *args)
+            self._keys_for = None
+        else:
+            self._ensure_sort_buffers(grid)
+            _, ws, gs = self._sort_ws
+            self.call("chm_push_x_keys", *args, gs, ptr(d["indx_in_cell"]), ptr(ws), ws.numel())
+            self._keys_for = grid
         self.flag_sorted = False
+        self.version += 1
 
     # ---------------------------------------------------------------- sort
     def _ensure_sort_buffers(self, grid):
@@ -101,9 +115,7 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         self.DataDev["sort_indx"] = torch.empty(Np, dtype=torch.int32, device=dev)
         self.DataDev["cell_offset"] = torch.empty(nbins + 1, dtype=torch.int32, device=dev)
         self.DataDev["Np_stay"] = torch.zeros(1, dtype=torch.int32, device=dev)
-        foff = _lib.lib.chm_sort_flags_offset(Np, nbins)
         self.DataDev["sum_in_cell"] = ws[: nbins * 4].view(torch.int32)
-        self._sort_flags = ws[foff: foff + 4].view(torch.int32)
         self._sort_ws = (key, ws, _grid_struct(grid))
 
     def index_sort(self, grid):
@@ -113,15 +125,18 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         self._ensure_sort_buffers(grid)
         _, ws, gs = self._sort_ws
         d = self.DataDev
-        self.call("chm_index_sort", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), self.Args["Np"],
-                  gs, ptr(d["indx_in_cell"]), ptr(d["cell_offset"]), ptr(d["sort_indx"]),
-                  ptr(d["Np_stay"]), ptr(ws), ws.numel())
+        if self._keys_for is grid:
+            _lib.call("chm_sort_from_keys", self.Args["Np"], gs, ptr(d["indx_in_cell"]),
+                      ptr(d["cell_offset"]), ptr(d["sort_indx"]), ptr(d["Np_stay"]), ptr(ws),
+                      ws.numel(), self.comm.stream())
+        else:
+            self.call("chm_index_sort", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), self.Args["Np"],
+                      gs, ptr(d["indx_in_cell"]), ptr(d["cell_offset"]), ptr(d["sort_indx"]),
+                      ptr(d["Np_stay"]), ptr(ws), ws.numel())
+        self._keys_for = None
+        self.version += 1
         dev_np_stay = d["Np_stay"]
         self.Args.set_lazy("Np_stay", lambda: int(dev_np_stay.item()))
-
-    def sort_flags(self):
-        """Sticky sort error flags (bit 0: a cell exceeded 16384 particles)."""
-        return 0 if self._sort_ws is None else int(self._sort_flags.item())
 
     # ---------------------------------------------------------------- align
     def align_and_damp(self, comps_align):
@@ -133,6 +148,7 @@ class ParticleMethodsHIP(GenericMethodsHIP):
             self.DataDev[comp] = self.DataDev[comp].index_select(0, idx).contiguous()
         self.reset_num_parts()
         self.flag_sorted = False
+        self.version += 1
 
     def free_mp(self):
         pass
@@ -145,6 +161,3 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         """Host copy of a particle component (float64 numpy)."""
         return self.DataDev[comp].detach().double().cpu().numpy()
 
-    def new_args(self, configs):
-        a = LazyArgs(configs)
-        return a

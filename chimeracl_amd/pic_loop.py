@@ -1,7 +1,8 @@
 """PIC_loop: the step orchestration of chimeraCL (chimeraCL/pic_loop.py:30-142).
 
-The order of operations is the reference's.  Three adjacent reference calls
-are fused into one kernel each, with identical arithmetic:
+The order of operations is the reference's.  Adjacent reference calls are
+fused, with identical arithmetic:
+  * push_coords + the cell keys of the following sort_parts -> one pass;
   * fb_transform(dir=0) of rho, J + fields_smooth -> filter applied in the
     transform's last pass;
   * the dN0 <- dN1 copy + field_grad -> one assembly kernel;
@@ -66,7 +67,7 @@ class PIC_loop:
         self.timer_record("frame")
         for parts in self.species:
             self.timer_start()
-            parts.push_coords(mode="half")
+            parts.push_coords(mode="half", grid=self.mainsolver)
             self.timer_record("push-x")
             self.timer_start()
             parts.sort_parts(grid=self.mainsolver)
@@ -77,7 +78,7 @@ class PIC_loop:
         self.timer_record("depose")
         for parts in self.species:
             self.timer_start()
-            parts.push_coords(mode="half")
+            parts.push_coords(mode="half", grid=self.mainsolver)
             self.timer_record("push-x")
             self.timer_start()
             parts.sort_parts(grid=self.mainsolver)
@@ -133,6 +134,3 @@ class PIC_loop:
                 self._body()
         torch.cuda.current_stream().wait_stream(s)
         self._graph = g
-        # the capture ran no kernels: undo the half pushes' bookkeeping only
-        for parts in self.species:
-            parts.flag_sorted = parts.flag_sorted

@@ -177,11 +177,26 @@ int chm_index_sort_f32(const float *x, const float *y, const float *z,
                        uint32_t *cell_offset, uint32_t *sort_indx,
                        uint32_t *np_stay, void *ws, size_t ws_bytes,
                        void *stream);
-/* Byte offset, inside the sort workspace, of the sticky uint32 error flags
- * (bit 0: a valid cell held more particles than the in-cell sort handles,
- * 16384; its slots were left in arrival order). */
-size_t chm_sort_flags_offset(int64_t Np, int64_t nbins);
-
+/* The two halves of index_sort, for the step's push-then-sort sequence
+ * (pic_loop.py:70-91, push_coords followed by sort_parts):
+ * chm_push_x_keys_* applies the coordinate push of chm_push_x_* and writes
+ * the cell keys and the histogram into the sort workspace in the same pass;
+ * chm_sort_from_keys then produces cell_offset, sort_indx and np_stay
+ * exactly as chm_index_sort_* does. */
+int chm_push_x_keys_f64(double *x, double *y, double *z, const double *px,
+                        const double *py, const double *pz,
+                        const double *g_inv, double dt, int64_t Np,
+                        const chm_grid_t *g, uint32_t *indx_in_cell, void *ws,
+                        size_t ws_bytes, void *stream);
+int chm_push_x_keys_f32(float *x, float *y, float *z, const float *px,
+                        const float *py, const float *pz, const float *g_inv,
+                        float dt, int64_t Np, const chm_grid_t *g,
+                        uint32_t *indx_in_cell, void *ws, size_t ws_bytes,
+                        void *stream);
+int chm_sort_from_keys(int64_t Np, const chm_grid_t *g,
+                       const uint32_t *indx_in_cell, uint32_t *cell_offset,
+                       uint32_t *sort_indx, uint32_t *np_stay, void *ws,
+                       size_t ws_bytes, void *stream);
 int chm_dens_profile_f64(const double *x, double *w, int64_t Np,
                          const double *xx, const double *ff,
                          const double *dxm1, int32_t nknots, void *stream);
@@ -220,6 +235,19 @@ int chm_deposit_vector_f32(const uint32_t *sort_indx,
                            const float *g_inv, const float *w, int32_t charge,
                            const chm_grid_t *g, const chm_planes_t *J,
                            void *ws, size_t ws_bytes, void *stream);
+/* The node pass of a deposit alone: adds to the planes the per-cell sums a
+ * previous chm_deposit_scalar_* (vector = 0) or chm_deposit_vector_*
+ * (vector = 1) call left in ws, in the same order.  For a species whose
+ * particles have not changed since (immobile ions: their per-cell sums are
+ * the same every step). */
+int chm_deposit_add_staged_f64(const uint32_t *cell_offset,
+                               const chm_grid_t *g, const chm_planes_t *flds,
+                               int32_t vector, void *ws, size_t ws_bytes,
+                               void *stream);
+int chm_deposit_add_staged_f32(const uint32_t *cell_offset,
+                               const chm_grid_t *g, const chm_planes_t *flds,
+                               int32_t vector, void *ws, size_t ws_bytes,
+                               void *stream);
 /* treat_axis (row1 -= row0) then *= dV_inv[ir], for nflds fields. */
 int chm_postproc_deposit_f64(const chm_planes_t *flds, int32_t nflds,
                              const double *dV_inv, int32_t Nx, int32_t Nr,

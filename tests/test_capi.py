@@ -38,7 +38,6 @@ def test_workspace_queries():
     from chimeracl_amd import _lib
     L = _lib.lib
     assert L.chm_sort_workspace_bytes(1000, 500) >= (500 + 4) * 4 * 2
-    assert L.chm_sort_flags_offset(1000, 500) < L.chm_sort_workspace_bytes(1000, 500)
     # deposit staging: 4 corners x components x cells, doubles
     assert L.chm_deposit_workspace_bytes(512, 128, 1, 1) >= 4 * 9 * 511 * 127 * 8
     assert L.chm_transform_workspace_bytes(512, 128, 1, 4, 8) >= 4 * 2 * 127 * 512 * 16

@@ -75,7 +75,6 @@ def test_push_sort_deposit_bit_exact(env, gi):
     assert np.array_equal(srt, np.argsort(key, kind="stable").astype(np.uint32))
     assert np.array_equal(srt, q.data["sort_indx"])
     assert p.Args["Np_stay"] == q.Args["Np_stay"]
-    assert p.sort_flags() == 0
     # deposits: bit-exact, including the node colour order
     solver.depose_charge([p])
     s.depose_charge([q])
@@ -108,6 +107,34 @@ def test_gather_push_bit_exact(env, gi):
         assert np.array_equal(p.host(k), q.data[k]), k
     for name in env.real_names(g["M"], flds=("E", "B"), scal=()):
         assert np.array_equal(env.field_host(solver, name), s.f[name]), name
+
+
+def test_sort_dense_cells_and_edges(env):
+    """Half of the particles in one cell, a quarter outside the domain, the
+    rest spread: keys, offsets and the stable order still match numpy; an
+    empty species sorts to empty arrays."""
+    g = env.tiny_grid(Nx=64, Nr=24, M=1)
+    solver, p, s, q = _pair_particles(env, g, 60000, 77, 1)
+    d = p.DataDev
+    n = 60000
+    x = d["x"].clone()
+    x[: n // 2] = g["Xmin"] + 10.5 * (g["Xmax"] - g["Xmin"]) / (g["Nx"] - 1)
+    x[n // 2: 3 * n // 4] = g["Xmax"] + 1.0
+    d["x"].copy_(x)
+    q.data["x"][:] = x.cpu().numpy()
+    p.sort_parts(solver)
+    q.sort_parts(s)
+    key = d["indx_in_cell"].cpu().numpy().view(np.uint32)
+    assert np.array_equal(key, q.data["indx_in_cell"])
+    assert np.array_equal(d["sort_indx"].cpu().numpy().view(np.uint32), np.argsort(key, kind="stable"))
+    assert np.array_equal(d["cell_offset"].cpu().numpy().view(np.uint32), q.data["cell_offset"])
+    assert p.Args["Np_stay"] == q.Args["Np_stay"]
+    import chimeracl_amd as CA
+    e = CA.Particles({"charge": -1, "dx": 0.02, "dr": 0.1}, solver.comm)
+    e.add_particles(source=type("S", (), {"DataDev": {k + "_new": np.zeros(0) for k in
+                                                      ("x", "y", "z", "px", "py", "pz", "w", "g_inv")}})())
+    e.sort_parts(solver)
+    assert e.Args["Np"] == 0
 
 
 def test_dens_profile_bit_exact(env):
