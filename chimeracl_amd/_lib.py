@@ -95,6 +95,8 @@ _SIGS = {
     "chm_tables_workspace_bytes": ([i32, i32, i32], sz),
     "chm_build_tables_f64": [i32, i32, i32, f64, f64, f64, f64, i32, R(TablesOut), P, sz, P],
     "chm_gemm_f64": [P, P, P, i32, i32, i32, P],
+    "chm_dht_table_elems": ([i32, i32], sz),
+    "chm_pad_table": [P, i32, i32, P, P, P, P, P],
     "chm_convert_table": [P, i64, P, P, P, P],
     "chm_laser_pulse_f64": [P, i32, i32, f64, f64, f64, f64, f64, f64, f64, f64, f64, P],
     "chm_laser_spectral_f64": [R(Spectral), P, P, P, f64, P],

@@ -2,14 +2,18 @@
 //
 //   C[M x N] = A[M x K] @ B[K x N]      (row-major, A = a DHT-type matrix)
 //
-// A is a per-mode real matrix [Nk][Nk] (DHT, DHT_inv, dDHT+-).  B is either a
-// real field [K][N] (mode 0), or a complex field [K][Nx] read as N = 2*Nx
-// real columns, optionally formed on the fly as alpha*X + beta*Y with either
-// operand read through the m = -1 map F_{-1}[k][x] = -conj(F_1[k][-x])
-// (get_m1, transformer_generic.cl:3-24).  This replaces Reikna's MatrixMul
+// A is a per-mode real matrix (DHT, DHT_inv, dDHT+-) stored zero-padded to
+// [round_up(M, 64)][lda], lda = round_up(K, 32), so A tiles load without
+// bounds checks in 16-byte vectors.  B is either a real field [K][N] (mode 0),
+// or a complex field [K][Nx] read as N = 2*Nx real columns, optionally formed
+// on the fly as alpha*X + beta*Y with either operand read through the
+// m = -1 map F_{-1}[k][x] = -conj(F_1[k][-x]) (get_m1,
+// transformer_generic.cl:3-24).  This replaces Reikna's MatrixMul
 // (transformer_methods_cl.py:458-472) and the axpy/get_m1 kernels that fed it.
 //
-// Three arithmetic forms:
+// Three arithmetic forms, 256 threads, 64 x 64 output tile per block, wave w
+// owning the 32 x 32 quadrant (w/2, w%2); the next K-step is loaded into
+// registers while the current one is multiplied:
 //   f64    : v_mfma_f64_16x16x4f64 (plain form)
 //   f32    : v_mfma_f32_16x16x4f32 (mixed form, CHM_DHT_F32)
 //   bf16x3 : v_mfma_f32_16x16x32_bf16 on split operands, A = Ah + Al,
@@ -21,77 +25,61 @@
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 #define BM 64
 #define BN 64
-#define APAD 16
 
 // ---------------------------------------------------------------- B loader
-// Loads 4 consecutive real columns n..n+3 of row k of B (zero outside).
+// Element B[k][n] (zero outside K x N).
 template <typename T>
-__device__ __forceinline__ void load_b4(const GemmBatch &gb, const GemmItem &it, int k, int n, T v[4]) {
-  if (k >= gb.K) {
-    v[0] = v[1] = v[2] = v[3] = 0;
-    return;
+__device__ __forceinline__ T load_b(const GemmBatch &gb, const GemmItem &it, int k, int n) {
+  if (k >= gb.K || n >= gb.N) return (T)0;
+  if (gb.bmode == 0) return ((const T *)it.X)[(int64_t)k * gb.ldb + n];
+  const int Nx = gb.Nx, x = n >> 1, h = n & 1;
+  T re = 0, im = 0;
+  {
+    int xs = it.remapX ? ((Nx - x) % Nx) : x;
+    const T *X = (const T *)it.X + 2 * ((int64_t)k * Nx + xs);
+    T a = X[0], b = X[1];
+    if (it.remapX) a = -a;  // -conj(F(-kx)): (-re, +im)
+    re = (T)it.alpha.x * a - (T)it.alpha.y * b;
+    im = (T)it.alpha.x * b + (T)it.alpha.y * a;
   }
-  if (gb.bmode == 0) {
-    const T *X = (const T *)it.X + (int64_t)k * gb.ldb;
-#pragma unroll
-    for (int q = 0; q < 4; q++) v[q] = (n + q < gb.N) ? X[n + q] : (T)0;
-    return;
+  if (it.Y) {
+    int xs = it.remapY ? ((Nx - x) % Nx) : x;
+    const T *Y = (const T *)it.Y + 2 * ((int64_t)k * Nx + xs);
+    T a = Y[0], b = Y[1];
+    if (it.remapY) a = -a;
+    re += (T)it.beta.x * a - (T)it.beta.y * b;
+    im += (T)it.beta.x * b + (T)it.beta.y * a;
   }
-  // complex: columns n, n+1 = (re, im) of x0 = n/2; n+2, n+3 of x0+1
-  const int Nx = gb.Nx;
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    int x = n / 2 + h;
-    T re = 0, im = 0;
-    if (x < Nx) {
-      {
-        int xs = it.remapX ? ((Nx - x) % Nx) : x;
-        const T *X = (const T *)it.X + 2 * ((int64_t)k * Nx + xs);
-        T a = X[0], b = X[1];
-        if (it.remapX) a = -a;  // -conj(F(-kx)): (-re, +im)
-        re = (T)it.alpha.x * a - (T)it.alpha.y * b;
-        im = (T)it.alpha.x * b + (T)it.alpha.y * a;
-      }
-      if (it.Y) {
-        int xs = it.remapY ? ((Nx - x) % Nx) : x;
-        const T *Y = (const T *)it.Y + 2 * ((int64_t)k * Nx + xs);
-        T a = Y[0], b = Y[1];
-        if (it.remapY) a = -a;
-        re += (T)it.beta.x * a - (T)it.beta.y * b;
-        im += (T)it.beta.x * b + (T)it.beta.y * a;
-      }
-    }
-    v[2 * h] = re;
-    v[2 * h + 1] = im;
-  }
+  return h ? im : re;
 }
 
 // ---------------------------------------------------------------- f64 / f32
-// 256 threads, tile BM x BN, BK = 16; wave w owns rows 32*(w/2), cols 32*(w%2).
 template <typename T>
 __global__ void __launch_bounds__(256) k_gemm(GemmBatch gb) {
   constexpr int BK = 16;
-  __shared__ T As[BK][BM + APAD];
-  __shared__ T Bs[BK][BN + APAD];
+  constexpr int AP = (sizeof(T) == 8) ? 8 : 16;  // row pad: stride = 128 B mod 256 B
+  __shared__ T As[BK][BM + AP];
+  __shared__ T Bs[BK][BN + AP];
   const GemmItem &it = gb.it[blockIdx.z];
   const int n0 = blockIdx.x * BN, m0 = blockIdx.y * BM;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const T *A = (const T *)it.A;
-  // global -> register staging
-  const int ar = tid >> 2, ak = (tid & 3) * 4;  // A: row ar, k ak..ak+3
-  const int bk = tid >> 4, bn = (tid & 15) * 4; // B: row bk, cols bn..bn+3
+  // A: thread loads row ar, k = ak..ak+3 (padded table, no checks)
+  const int ar = tid >> 2, ak = (tid & 3) * 4;
+  // B: thread owns column bn and rows bk, bk+4, bk+8, bk+12
+  const int bn = tid & 63, bk = tid >> 6;
   T ra[4], rb[4];
   auto gload = [&](int k0) {
+    const T *pa = A + (int64_t)(m0 + ar) * gb.lda + k0 + ak;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      int row = m0 + ar, k = k0 + ak + q;
-      ra[q] = (row < gb.M && k < gb.K) ? A[(int64_t)row * gb.lda + k] : (T)0;
-    }
-    load_b4<T>(gb, it, k0 + bk, n0 + bn, rb);
+    for (int q = 0; q < 4; q++) ra[q] = pa[q];
+#pragma unroll
+    for (int q = 0; q < 4; q++) rb[q] = load_b<T>(gb, it, k0 + bk + 4 * q, n0 + bn);
   };
   typedef typename std::conditional<sizeof(T) == 8, d4, f4>::type acc_t;
   acc_t acc[2][2];
@@ -104,7 +92,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmBatch gb) {
 #pragma unroll
     for (int q = 0; q < 4; q++) As[ak + q][ar] = ra[q];
 #pragma unroll
-    for (int q = 0; q < 4; q++) Bs[bk][bn + q] = rb[q];
+    for (int q = 0; q < 4; q++) Bs[bk + 4 * q][bn] = rb[q];
     __syncthreads();
     if (k0 + BK < gb.K) gload(k0 + BK);
 #pragma unroll
@@ -142,66 +130,78 @@ __global__ void __launch_bounds__(256) k_gemm(GemmBatch gb) {
 }
 
 // ---------------------------------------------------------------- split-BF16
-__device__ __forceinline__ void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
-  hi = (__bf16)v;
-  lo = (__bf16)(v - (float)hi);
+__device__ __forceinline__ void split_bf16(float v, unsigned short &hi, unsigned short &lo) {
+  __bf16 h = (__bf16)v;
+  __bf16 l = (__bf16)(v - (float)h);
+  hi = __builtin_bit_cast(unsigned short, h);
+  lo = __builtin_bit_cast(unsigned short, l);
 }
 
 __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
   constexpr int BK = 32;
+  constexpr int KP = BK + 8;  // row stride 80 B: 16-byte aligned, staggers banks
   // operands stored k-contiguous so each lane reads 8 bf16 (16 B) at once
-  __shared__ __attribute__((aligned(16))) __bf16 Ah[BM][BK + 8], Al[BM][BK + 8];
-  __shared__ __attribute__((aligned(16))) __bf16 Bh[BN][BK + 8], Bl[BN][BK + 8];
+  __shared__ __attribute__((aligned(16))) unsigned short Ah[BM][KP], Al[BM][KP];
+  __shared__ __attribute__((aligned(16))) unsigned short Bh[BN][KP], Bl[BN][KP];
   const GemmItem &it = gb.it[blockIdx.z];
   const int n0 = blockIdx.x * BN, m0 = blockIdx.y * BM;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
-  const uint16_t *Ahg = (const uint16_t *)it.A, *Alg = (const uint16_t *)it.A_lo;
+  const unsigned short *Ahg = (const unsigned short *)it.A, *Alg = (const unsigned short *)it.A_lo;
+  // A: thread loads row ar, k = ak..ak+7 as one 16-byte vector (hi and lo)
+  const int ar = tid >> 2, ak = (tid & 3) * 8;
+  // B: thread owns column bn and k = bk..bk+7
+  const int bn = tid & 63, bk = (tid >> 6) * 8;
+  us8 rah, ral;
+  float rb[8];
+  auto gload = [&](int k0) {
+    int64_t o = (int64_t)(m0 + ar) * gb.lda + k0 + ak;
+    rah = *(const us8 *)(Ahg + o);
+    ral = *(const us8 *)(Alg + o);
+#pragma unroll
+    for (int q = 0; q < 8; q++) rb[q] = load_b<float>(gb, it, k0 + bk + q, n0 + bn);
+  };
   f4 acc[2][2], cor[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = cor[i][j] = (f4){0, 0, 0, 0};
-  const int ar = tid >> 2, ak = (tid & 3) * 8;  // A: row ar, k ak..ak+7
+  gload(0);
   for (int k0 = 0; k0 < gb.K; k0 += BK) {
+    *(us8 *)&Ah[ar][ak] = rah;
+    *(us8 *)&Al[ar][ak] = ral;
+    us8 bh, bl;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      int row = m0 + ar, k = k0 + ak + q;
-      bool ok = row < gb.M && k < gb.K;
-      uint16_t h = ok ? Ahg[(int64_t)row * gb.lda + k] : 0, l = ok ? Alg[(int64_t)row * gb.lda + k] : 0;
-      Ah[ar][ak + q] = __builtin_bit_cast(__bf16, h);
-      Al[ar][ak + q] = __builtin_bit_cast(__bf16, l);
+      unsigned short h, l;
+      split_bf16(rb[q], h, l);
+      bh[q] = h;
+      bl[q] = l;
     }
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-      int t = tid + 256 * s;
-      int bk = t >> 4, bn = (t & 15) * 4;
-      float v[4];
-      load_b4<float>(gb, it, k0 + bk, n0 + bn, v);
-#pragma unroll
-      for (int q = 0; q < 4; q++) split_bf16(v[q], Bh[bn + q][bk], Bl[bn + q][bk]);
-    }
+    *(us8 *)&Bh[bn][bk] = bh;
+    *(us8 *)&Bl[bn][bk] = bl;
     __syncthreads();
-    bf8 ah[2], al[2], bh[2], bl[2];
+    if (k0 + BK < gb.K) gload(k0 + BK);
+    bf8 ah[2], al[2], bhv[2], blv[2];
 #pragma unroll
     for (int i = 0; i < 2; i++) {
       int row = 32 * wr + 16 * i + (lane & 15), k = 8 * (lane >> 4);
-      ah[i] = *(const bf8 *)&Ah[row][k];
-      al[i] = *(const bf8 *)&Al[row][k];
+      ah[i] = __builtin_bit_cast(bf8, *(const us8 *)&Ah[row][k]);
+      al[i] = __builtin_bit_cast(bf8, *(const us8 *)&Al[row][k]);
     }
 #pragma unroll
     for (int j = 0; j < 2; j++) {
       int col = 32 * wc + 16 * j + (lane & 15), k = 8 * (lane >> 4);
-      bh[j] = *(const bf8 *)&Bh[col][k];
-      bl[j] = *(const bf8 *)&Bl[col][k];
+      bhv[j] = __builtin_bit_cast(bf8, *(const us8 *)&Bh[col][k]);
+      blv[j] = __builtin_bit_cast(bf8, *(const us8 *)&Bl[col][k]);
     }
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
       for (int j = 0; j < 2; j++) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-        cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], cor[i][j], 0, 0, 0);
-        cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], cor[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bhv[j], acc[i][j], 0, 0, 0);
+        cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], blv[j], cor[i][j], 0, 0, 0);
+        cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bhv[j], cor[i][j], 0, 0, 0);
       }
     __syncthreads();
   }
@@ -221,6 +221,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
 int chm_gemm_launch(const GemmBatch &gb, int nitems, int kind, hipStream_t st) {
   if (nitems <= 0) return CHM_OK;
   if (nitems > GEMM_MAXB || gb.M <= 0 || gb.K <= 0 || gb.N <= 0) return CHM_EINVAL;
+  if (gb.lda < gemm_lda(gb.K)) return CHM_EINVAL;
   dim3 grid((gb.N + BN - 1) / BN, (gb.M + BM - 1) / BM, nitems);
   if (kind == GEMM_F64) k_gemm<double><<<grid, 256, 0, st>>>(gb);
   else if (kind == GEMM_F32) k_gemm<float><<<grid, 256, 0, st>>>(gb);
@@ -229,15 +230,50 @@ int chm_gemm_launch(const GemmBatch &gb, int nitems, int kind, hipStream_t st) {
   return CHM_OK;
 }
 
+// Zero-padded copy of a row-major [M][K] double matrix into the layout the
+// products read: [round_up(M, 64)][gemm_lda(K)] as double, float or split BF16.
+__global__ void k_pad_table(const double *__restrict__ src, int M, int K, int Mp, int lda, double *d64,
+                            float *f32, unsigned short *hi, unsigned short *lo) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)Mp * lda) return;
+  int i = (int)(e / lda), k = (int)(e % lda);
+  double v = (i < M && k < K) ? src[(int64_t)i * K + k] : 0.0;
+  if (d64) d64[e] = v;
+  if (f32) f32[e] = (float)v;
+  if (hi) {
+    __bf16 h = (__bf16)(float)v;
+    __bf16 l = (__bf16)(float)(v - (double)(float)h);
+    hi[e] = __builtin_bit_cast(unsigned short, h);
+    lo[e] = __builtin_bit_cast(unsigned short, l);
+  }
+}
+
+extern "C" size_t chm_dht_table_elems(int32_t M, int32_t K) {
+  return (size_t)((M + BM - 1) / BM * BM) * gemm_lda(K);
+}
+
+extern "C" int chm_pad_table(const double *src, int32_t M, int32_t K, double *d64, float *f32,
+                             uint16_t *bf16_hi, uint16_t *bf16_lo, void *stream) {
+  if (!src || M <= 0 || K <= 0 || ((bf16_hi == nullptr) != (bf16_lo == nullptr))) return CHM_EINVAL;
+  int Mp = (M + BM - 1) / BM * BM, lda = gemm_lda(K);
+  int64_t n = (int64_t)Mp * lda;
+  k_pad_table<<<chm_blocks(n, 256), 256, 0, (hipStream_t)stream>>>(src, M, K, Mp, lda, d64, f32,
+                                                                   (unsigned short *)bf16_hi,
+                                                                   (unsigned short *)bf16_lo);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
 // Plain real product for table construction and tests:
-// C[M x N] = A[M x K] @ B[K x N], all double, row-major, contiguous.
+// C[M x N] = A[M x K] @ B[K x N]; A padded as chm_pad_table writes it
+// ([round_up(M,64)][round_up(K,32)]), B and C row-major, contiguous.
 extern "C" int chm_gemm_f64(const double *A, const double *B, double *C, int32_t M, int32_t K,
                             int32_t N, void *stream) {
   GemmBatch gb = {};
   gb.M = M;
   gb.K = K;
   gb.N = N;
-  gb.lda = K;
+  gb.lda = gemm_lda(K);
   gb.ldb = N;
   gb.ldc = N;
   gb.bmode = 0;

@@ -138,7 +138,7 @@ static int transform(const chm_spectral_t *sp, const chm_planes_t *real, const c
         gb.M = Nk;
         gb.K = Nk;
         gb.N = m == 0 ? Nx : 2 * Nx;
-        gb.lda = Nk;
+        gb.lda = gemm_lda(Nk);
         gb.ldb = gb.N;
         gb.ldc = gb.N;
         gb.bmode = 0;
@@ -185,7 +185,7 @@ static int transform(const chm_spectral_t *sp, const chm_planes_t *real, const c
       gb.M = Nk;
       gb.K = Nk;
       gb.N = m == 0 ? Nx : 2 * Nx;
-      gb.lda = Nk;
+      gb.lda = gemm_lda(Nk);
       gb.ldb = gb.N;
       gb.ldc = gb.N;
       gb.bmode = 0;
@@ -359,7 +359,7 @@ static int spectral_op(int OP, const chm_spectral_t *sp, const chm_fb_t *in, con
   gb.K = Nk;
   gb.N = 2 * Nx;
   gb.Nx = Nx;
-  gb.lda = Nk;
+  gb.lda = gemm_lda(Nk);
   gb.ldc = 2 * Nx;
   gb.bmode = 1;
   int ni = 0, ns = 0;

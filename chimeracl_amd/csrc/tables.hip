@@ -185,7 +185,8 @@ __global__ void k_damp_profile(double *f, int N) {
 
 extern "C" size_t chm_tables_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M) {
   size_t Nk = (size_t)(Nr - 1);
-  return chm_align(Nk * 2 * Nk * 8) + 2 * chm_align(Nk * Nk * 8);
+  return chm_align(Nk * 2 * Nk * 8) + 2 * chm_align(Nk * Nk * 8) +
+         chm_align(chm_dht_table_elems((int32_t)Nk, (int32_t)Nk) * 8);
 }
 
 extern "C" int chm_build_tables_f64(int32_t Nx, int32_t Nr, int32_t M, double dx, double dr, double dt,
@@ -201,6 +202,7 @@ extern "C" int chm_build_tables_f64(int32_t Nx, int32_t Nr, int32_t M, double dx
   double *aug = (double *)b;
   double *Jp = (double *)(b + chm_align((size_t)Nk * 2 * Nk * 8));
   double *Jm = (double *)((char *)Jp + chm_align((size_t)Nk * Nk * 8));
+  double *Dpad = (double *)((char *)Jm + chm_align((size_t)Nk * Nk * 8));
   k_kx<<<chm_blocks(Nx, 256), 256, 0, st>>>(out->kx, Nx, dx);
   k_roots<<<chm_blocks((M + 2) * Nk, 64), 64, 0, st>>>(out->kr, M + 2, Nk, R_period);
   int64_t nn = (int64_t)Nk * Nk;
@@ -212,9 +214,11 @@ extern "C" int chm_build_tables_f64(int32_t Nx, int32_t Nr, int32_t M, double dx
     k_gauss_jordan<<<1, 1024, 0, st>>>(out->DHT_inv[m], out->DHT[m], aug, Nk);
     k_jmat<<<chm_blocks(nn, 256), 256, 0, st>>>(Jp, krp, m, Nk, rmin, dr, 1);
     k_jmat<<<chm_blocks(nn, 256), 256, 0, st>>>(Jm, krm, m, Nk, rmin, dr, 1);
-    int rc = chm_gemm_f64(out->DHT[m], Jp, out->dDp[m], Nk, Nk, Nk, stream);
+    int rc = chm_pad_table(out->DHT[m], Nk, Nk, Dpad, nullptr, nullptr, nullptr, stream);
     if (rc) return rc;
-    rc = chm_gemm_f64(out->DHT[m], Jm, out->dDm[m], Nk, Nk, Nk, stream);
+    rc = chm_gemm_f64(Dpad, Jp, out->dDp[m], Nk, Nk, Nk, stream);
+    if (rc) return rc;
+    rc = chm_gemm_f64(Dpad, Jm, out->dDm[m], Nk, Nk, Nk, stream);
     if (rc) return rc;
     k_wtables<<<chm_blocks((int64_t)Nk * Nx, 256), 256, 0, st>>>(out->kx, kr0, Nx, Nk, dt, out->w[m], out->c1[m],
                                                                  out->c2[m], out->c3[m], out->smooth[m]);

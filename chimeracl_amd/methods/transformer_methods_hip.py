@@ -78,8 +78,10 @@ class TransformerMethodsHIP(GenericMethodsHIP):
         a, D, dev = self.Args, self.DataDev, self.comm.device
         M = a["M"]
         sp = _lib.Spectral(Nx=a["Nx"], Nr=a["Nr"], M=M, ndamp=2 * nd, dt_inv=a["dt_inv"])
+        Nk = a["Nr"] - 1
+        nel = _lib.lib.chm_dht_table_elems(Nk, Nk)
         if not self.comm.mixed:
-            src = D
+            src = dict(D)
         else:
             # FP32 copies of every table; DHT-type matrices also split in BF16
             src = {}
@@ -90,18 +92,25 @@ class TransformerMethodsHIP(GenericMethodsHIP):
                 src[key] = D[key].float()
             for key in ("phase_minus", "phase_plus", "fft_twiddle"):
                 src[key] = D[key].to(torch.complex64)
-            for m in range(M + 1):
-                for key in ("DHT_m", "DHT_inv_m", "dDHT_plus_m", "dDHT_minus_m"):
-                    T = D[key + str(m)]
-                    if self.comm.dht_kind == _lib.CHM_DHT_BF16X3:
-                        hi = torch.empty(T.shape, dtype=torch.int16, device=dev)
-                        lo = torch.empty(T.shape, dtype=torch.int16, device=dev)
-                        _lib.call("chm_convert_table", ptr(T), T.numel(), None, ptr(hi), ptr(lo),
-                                  self.comm.stream())
-                        src[key + str(m)], src[key + str(m) + "_lo"] = hi, lo
-                    else:
-                        src[key + str(m)] = T.float()
-            self._mixed_tables = src
+        # DHT-type matrices in the zero-padded layout the MFMA products read
+        st = self.comm.stream()
+        for m in range(M + 1):
+            for key in ("DHT_m", "DHT_inv_m", "dDHT_plus_m", "dDHT_minus_m"):
+                T = D[key + str(m)]
+                if not self.comm.mixed:
+                    pad = torch.empty(nel, dtype=torch.float64, device=dev)
+                    _lib.call("chm_pad_table", ptr(T), Nk, Nk, ptr(pad), None, None, None, st)
+                    src[key + str(m)] = pad
+                elif self.comm.dht_kind == _lib.CHM_DHT_BF16X3:
+                    hi = torch.empty(nel, dtype=torch.int16, device=dev)
+                    lo = torch.empty(nel, dtype=torch.int16, device=dev)
+                    _lib.call("chm_pad_table", ptr(T), Nk, Nk, None, None, ptr(hi), ptr(lo), st)
+                    src[key + str(m)], src[key + str(m) + "_lo"] = hi, lo
+                else:
+                    pad = torch.empty(nel, dtype=torch.float32, device=dev)
+                    _lib.call("chm_pad_table", ptr(T), Nk, Nk, None, ptr(pad), None, None, st)
+                    src[key + str(m)] = pad
+        self._kernel_tables = src
         sp.kx = ptr(src["kx"])
         sp.phase_minus, sp.phase_plus = ptr(src["phase_minus"]), ptr(src["phase_plus"])
         sp.damp = ptr(src["DampProfile"]) if nd > 0 else None

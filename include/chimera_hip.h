@@ -349,8 +349,19 @@ int chm_build_tables_f64(int32_t Nx, int32_t Nr, int32_t M, double dx,
                          double dr, double dt, double xmin, int32_t damp_cells,
                          const chm_tables_out_t *out, void *ws,
                          size_t ws_bytes, void *stream);
-/* Plain FP64 product C[M x N] = A[M x K] @ B[K x N] on the matrix cores
- * (row-major, contiguous); used for the dDHT tables and by the tests. */
+/* The DHT-type matrices of chm_spectral_t (DHT, DHT_inv, dDp, dDm and their
+ * _lo parts) are read in a zero-padded layout: [round_up(Nr-1, 64)] rows of
+ * round_up(Nr-1, 32) elements.  chm_dht_table_elems gives its size in
+ * elements; chm_pad_table writes it from a row-major [M][K] FP64 matrix as
+ * double (d64), float (f32) and/or a split-BF16 pair (bf16_hi + bf16_lo ~=
+ * src); any output may be NULL (hi and lo together). */
+size_t chm_dht_table_elems(int32_t M, int32_t K);
+int chm_pad_table(const double *src, int32_t M, int32_t K, double *d64,
+                  float *f32, uint16_t *bf16_hi, uint16_t *bf16_lo,
+                  void *stream);
+/* Plain FP64 product C[M x N] = A[M x K] @ B[K x N] on the matrix cores,
+ * A in the padded layout above, B and C row-major and contiguous; used for
+ * the dDHT tables and by the tests. */
 int chm_gemm_f64(const double *A, const double *B, double *C, int32_t M,
                  int32_t K, int32_t N, void *stream);
 /* Converts n doubles to float (round to nearest) and, if hi/lo are not NULL,
