@@ -100,6 +100,35 @@ _SIGS = {
     "chm_convert_table": [P, i64, P, P, P, P],
     "chm_laser_pulse_f64": [P, i32, i32, f64, f64, f64, f64, f64, f64, f64, f64, f64, P],
     "chm_laser_spectral_f64": [R(Spectral), P, P, P, f64, P],
+    # adjoints (differentiable form)
+    "chm_push_x_adj_f64": [P, P, P, P, P, P, P, P, P, P, P, f64, i64, P],
+    "chm_push_x_adj_f32": [P, P, P, P, P, P, P, P, P, P, P, f32, i64, P],
+    "chm_deposit_scalar_adj_f64": [P, P, P, P, i32, i64, R(Grid), R(Planes), P, P, P, P, P],
+    "chm_deposit_scalar_adj_f32": [P, P, P, P, i32, i64, R(Grid), R(Planes), P, P, P, P, P],
+    "chm_deposit_vector_adj_f64": [P, P, P, P, P, P, P, P, i32, i64, R(Grid), R(Planes),
+                                   P, P, P, P, P, P, P, P, P],
+    "chm_deposit_vector_adj_f32": [P, P, P, P, P, P, P, P, i32, i64, R(Grid), R(Planes),
+                                   P, P, P, P, P, P, P, P, P],
+    "chm_gather_push_adj_f64": [P, P, P, P, P, P, i64, P, f64, R(Grid), R(Planes), R(Planes),
+                                P, P, P, P, P, P, P, P, P],
+    "chm_gather_push_adj_f32": [P, P, P, P, P, P, i64, P, f32, R(Grid), R(Planes), R(Planes),
+                                P, P, P, P, P, P, P, P, P],
+    "chm_gather_adj_scatter_f64": [P, P, P, P, P, P, P, P, R(Grid), R(Planes), P, sz, P],
+    "chm_gather_adj_scatter_f32": [P, P, P, P, P, P, P, P, R(Grid), R(Planes), P, sz, P],
+    "chm_postproc_deposit_adj_f64": [R(Planes), i32, P, i32, i32, i32, P],
+    "chm_postproc_deposit_adj_f32": [R(Planes), i32, P, i32, i32, i32, P],
+    "chm_warp_axis_adj_f64": [R(Planes), i32, i32, i32, i32, P],
+    "chm_warp_axis_adj_f32": [R(Planes), i32, i32, i32, i32, P],
+    "chm_profile_adj_workspace_bytes": ([i32], sz),
+    "chm_dens_profile_adj_f64": [P, P, P, i64, P, P, i32, P, P, sz, P],
+    "chm_dens_profile_adj_f32": [P, P, P, i64, P, P, i32, P, P, sz, P],
+    "chm_field_grad_adj_f64": [R(Spectral), R(Fb), R(Fb), i32, P, sz, P],
+    "chm_field_grad_adj_f32": [R(Spectral), R(Fb), R(Fb), i32, P, sz, P],
+    "chm_field_rot_adj_f64": [R(Spectral), R(Fb), R(Fb), i32, i32, P, sz, P],
+    "chm_field_rot_adj_f32": [R(Spectral), R(Fb), R(Fb), i32, i32, P, sz, P],
+    "chm_damp_fields_adj_f64": [R(Spectral), R(Fb), R(Fb), R(Fb), R(Fb), R(Fb), R(Fb), P],
+    "chm_damp_fields_adj_f32": [R(Spectral), R(Fb), R(Fb), R(Fb), R(Fb), R(Fb), R(Fb), P],
+    "chm_laser_pulse_dR_f64": [P, i32, i32, f64, f64, f64, f64, f64, f64, f64, f64, f64, P],
 }
 
 ERRORS = {-1: "CHM_EINVAL (argument out of range)", -2: "CHM_EWORKSPACE (workspace too small)",

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchimera_hip.so")
-SOURCES = ["particles.hip", "deposit.hip", "gemm.hip", "spectral.hip", "tables.hip"]
+SOURCES = ["particles.hip", "deposit.hip", "gemm.hip", "spectral.hip", "tables.hip",
+           "adjoint.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CHM_OFFLOAD_ARCH", "gfx950")
 

@@ -42,15 +42,18 @@ struct PlanePtrs {
 // spreads them over the LDS banks.
 #define DEP_SW(t) ((t) + ((t) >> 5))
 
-template <typename T, int M, bool VEC>
+// KIND 0: rho (wp = w q); 1: J (wp = w g_inv q, values u); 2: field
+// adjoints of the gather (wp = 1, values in ux/uy/uz, modes weighted 2 e_m).
+template <typename T, int M, int KIND>
 __global__ void __launch_bounds__(64)
     k_cell_sums(const uint32_t *__restrict__ sort_indx, const uint32_t *__restrict__ cell_offset,
                 const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
                 const T *__restrict__ ux, const T *__restrict__ uy, const T *__restrict__ uz,
                 const T *__restrict__ g_inv, const T *__restrict__ w, T charge, int Nx, int Nr,
                 T xmin, T dx_inv, T rmin, T dr_inv, T *__restrict__ part) {
+  constexpr bool VEC = KIND != 0;
   constexpr int NK = VEC ? 3 : 1;
-  constexpr int NV = VEC ? 8 : 4;
+  constexpr int NV = KIND == 1 ? 8 : (KIND == 2 ? 6 : 4);
   __shared__ T st[NV][DEP_CAP + DEP_CAP / 32];
   const int Nx_cell = Nx - 1, Nr_cell = Nr - 1;
   const int64_t ncell = (int64_t)Nx_cell * Nr_cell;
@@ -90,8 +93,14 @@ __global__ void __launch_bounds__(64)
       st[0][q] = x[p];
       st[1][q] = y[p];
       st[2][q] = z[p];
-      st[3][q] = w[p];
-      if (VEC) {
+      if (KIND == 2) {
+        st[3][q] = ux[p];
+        st[4][q] = uy[p];
+        st[5][q] = uz[p];
+      } else {
+        st[3][q] = w[p];
+      }
+      if (KIND == 1) {
         st[4][q] = ux[p];
         st[5][q] = uy[p];
         st[6][q] = uz[p];
@@ -106,7 +115,12 @@ __global__ void __launch_bounds__(64)
         T xp = st[0][t], yp = st[1][t], zp = st[2][t];
         T jp[3];
         T wp;
-        if (VEC) {
+        if (KIND == 2) {
+          jp[0] = st[3][t];
+          jp[1] = st[4][t];
+          jp[2] = st[5][t];
+          wp = (T)1;
+        } else if (KIND == 1) {
           jp[0] = st[4][t];
           jp[1] = st[5][t];
           jp[2] = st[6][t];
@@ -130,6 +144,13 @@ __global__ void __launch_bounds__(64)
         for (int m = 2; m <= M; m++) {
           er[m] = er[m - 1] * er[1] - ei[m - 1] * ei[1];
           ei[m] = er[m - 1] * ei[1] + ei[m - 1] * er[1];
+        }
+        if (KIND == 2) {
+#pragma unroll
+          for (int m = 1; m <= M; m++) {
+            er[m] = (T)2 * er[m];
+            ei[m] = (T)2 * ei[m];
+          }
         }
         T sX1 = (xp - xmin) * dx_inv - (T)ix;
         T sX0 = (T)1.0 - sX1;
@@ -245,30 +266,30 @@ extern "C" size_t chm_deposit_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M,
   return chm_align(nq * ncell * sizeof(double));
 }
 
-template <typename T, int M, bool VEC>
+template <typename T, int M, int KIND>
 static void launch_deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const T *x,
                            const T *y, const T *z, const T *ux, const T *uy, const T *uz,
                            const T *g_inv, const T *w, T charge, const chm_grid_t *g,
                            const PlanePtrs &pp, T *part, bool pass1, hipStream_t st) {
   int64_t ncell = (int64_t)(g->Nx - 1) * (g->Nr - 1);
   if (pass1)
-    k_cell_sums<T, M, VEC><<<chm_blocks(ncell, DEP_CELLS), 64, 0, st>>>(
+    k_cell_sums<T, M, KIND><<<chm_blocks(ncell, DEP_CELLS), 64, 0, st>>>(
       sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr, (T)g->xmin,
       (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, part);
   int64_t nnode = (int64_t)(g->Nr - 1) * g->Nx;
-  k_node_add<T, M, VEC ? 3 : 1><<<chm_blocks(nnode, 256), 256, 0, st>>>(cell_offset, part, g->Nx,
+  k_node_add<T, M, KIND ? 3 : 1><<<chm_blocks(nnode, 256), 256, 0, st>>>(cell_offset, part, g->Nx,
                                                                         g->Nr, pp);
 }
 
-template <typename T, bool VEC>
+template <typename T, int KIND>
 static int deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const T *x, const T *y,
                    const T *z, const T *ux, const T *uy, const T *uz, const T *g_inv, const T *w,
                    int32_t charge, const chm_grid_t *g, const chm_planes_t *flds, void *ws,
                    size_t ws_bytes, void *stream, bool pass1 = true) {
   if (!g || !flds || g->M < 0 || g->M > CHM_MAXM || g->Nx < 4 || g->Nr < 3) return CHM_EINVAL;
-  if (chm_deposit_workspace_bytes(g->Nx, g->Nr, g->M, VEC) > ws_bytes || !ws) return CHM_EWORKSPACE;
+  if (chm_deposit_workspace_bytes(g->Nx, g->Nr, g->M, KIND != 0) > ws_bytes || !ws) return CHM_EWORKSPACE;
   PlanePtrs pp = {};
-  for (int k = 0; k < (VEC ? 3 : 1); k++) {
+  for (int k = 0; k < (KIND ? 3 : 1); k++) {
     pp.m0[k] = flds[k].m0;
     for (int m = 0; m < g->M; m++) pp.mc[k][m] = flds[k].mc[m];
   }
@@ -279,7 +300,7 @@ static int deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const
   switch (g->M) {
 #define CASE(MM)                                                                                   \
   case MM:                                                                                         \
-    launch_deposit<T, MM, VEC>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, q, g, pp,    \
+    launch_deposit<T, MM, KIND>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, q, g, pp,   \
                                part, pass1, st);                                                   \
     break;
     CASE(0) CASE(1) CASE(2) CASE(3) CASE(4)
@@ -294,7 +315,7 @@ extern "C" int chm_deposit_scalar_f64(const uint32_t *sort_indx, const uint32_t 
                                       const double *w, int32_t charge, const chm_grid_t *g,
                                       const chm_planes_t *rho, void *ws, size_t ws_bytes,
                                       void *stream) {
-  return deposit<double, false>(sort_indx, cell_offset, x, y, z, nullptr, nullptr, nullptr, nullptr,
+  return deposit<double, 0>(sort_indx, cell_offset, x, y, z, nullptr, nullptr, nullptr, nullptr,
                                 w, charge, g, rho, ws, ws_bytes, stream);
 }
 extern "C" int chm_deposit_scalar_f32(const uint32_t *sort_indx, const uint32_t *cell_offset,
@@ -302,7 +323,7 @@ extern "C" int chm_deposit_scalar_f32(const uint32_t *sort_indx, const uint32_t 
                                       const float *w, int32_t charge, const chm_grid_t *g,
                                       const chm_planes_t *rho, void *ws, size_t ws_bytes,
                                       void *stream) {
-  return deposit<float, false>(sort_indx, cell_offset, x, y, z, nullptr, nullptr, nullptr, nullptr,
+  return deposit<float, 0>(sort_indx, cell_offset, x, y, z, nullptr, nullptr, nullptr, nullptr,
                                w, charge, g, rho, ws, ws_bytes, stream);
 }
 extern "C" int chm_deposit_vector_f64(const uint32_t *sort_indx, const uint32_t *cell_offset,
@@ -311,7 +332,7 @@ extern "C" int chm_deposit_vector_f64(const uint32_t *sort_indx, const uint32_t 
                                       const double *g_inv, const double *w, int32_t charge,
                                       const chm_grid_t *g, const chm_planes_t *J, void *ws,
                                       size_t ws_bytes, void *stream) {
-  return deposit<double, true>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g, J,
+  return deposit<double, 1>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g, J,
                                ws, ws_bytes, stream);
 }
 extern "C" int chm_deposit_vector_f32(const uint32_t *sort_indx, const uint32_t *cell_offset,
@@ -320,8 +341,29 @@ extern "C" int chm_deposit_vector_f32(const uint32_t *sort_indx, const uint32_t 
                                       const float *g_inv, const float *w, int32_t charge,
                                       const chm_grid_t *g, const chm_planes_t *J, void *ws,
                                       size_t ws_bytes, void *stream) {
-  return deposit<float, true>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g, J,
+  return deposit<float, 1>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g, J,
                               ws, ws_bytes, stream);
+}
+
+// Scatter of the gather's per-particle field adjoints (three components,
+// e.g. those of E_p) onto the adjoint planes: planes += sum_p C_ij v_p
+// (mode 0) and C_ij v_p 2 e_m (modes m >= 1), the transpose of the gather's
+// interpolation.  Same two passes and workspace as the vector deposit.
+extern "C" int chm_gather_adj_scatter_f64(const uint32_t *sort_indx, const uint32_t *cell_offset,
+                                          const double *x, const double *y, const double *z,
+                                          const double *v0, const double *v1, const double *v2,
+                                          const chm_grid_t *g, const chm_planes_t *out, void *ws,
+                                          size_t ws_bytes, void *stream) {
+  return deposit<double, 2>(sort_indx, cell_offset, x, y, z, v0, v1, v2, nullptr, nullptr, 1, g,
+                            out, ws, ws_bytes, stream);
+}
+extern "C" int chm_gather_adj_scatter_f32(const uint32_t *sort_indx, const uint32_t *cell_offset,
+                                          const float *x, const float *y, const float *z,
+                                          const float *v0, const float *v1, const float *v2,
+                                          const chm_grid_t *g, const chm_planes_t *out, void *ws,
+                                          size_t ws_bytes, void *stream) {
+  return deposit<float, 2>(sort_indx, cell_offset, x, y, z, v0, v1, v2, nullptr, nullptr, 1, g,
+                           out, ws, ws_bytes, stream);
 }
 
 // Second pass alone: adds the per-cell sums a previous deposit call left in
@@ -330,18 +372,18 @@ extern "C" int chm_deposit_add_staged_f64(const uint32_t *cell_offset, const chm
                                           const chm_planes_t *flds, int32_t vector, void *ws,
                                           size_t ws_bytes, void *stream) {
   if (vector)
-    return deposit<double, true>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
+    return deposit<double, 1>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
                                  nullptr, nullptr, nullptr, 0, g, flds, ws, ws_bytes, stream, false);
-  return deposit<double, false>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
+  return deposit<double, 0>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
                                 nullptr, nullptr, nullptr, 0, g, flds, ws, ws_bytes, stream, false);
 }
 extern "C" int chm_deposit_add_staged_f32(const uint32_t *cell_offset, const chm_grid_t *g,
                                           const chm_planes_t *flds, int32_t vector, void *ws,
                                           size_t ws_bytes, void *stream) {
   if (vector)
-    return deposit<float, true>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
+    return deposit<float, 1>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
                                 nullptr, nullptr, nullptr, 0, g, flds, ws, ws_bytes, stream, false);
-  return deposit<float, false>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
+  return deposit<float, 0>(nullptr, cell_offset, nullptr, nullptr, nullptr, nullptr, nullptr,
                                nullptr, nullptr, nullptr, 0, g, flds, ws, ws_bytes, stream, false);
 }
 

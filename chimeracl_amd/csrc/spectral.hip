@@ -533,6 +533,47 @@ extern "C" int chm_advance_fields_f32(const chm_spectral_t *sp, const chm_fb_t *
 // One block per (row kr, mode m, component c): optional PSATD of the row,
 // then for E and G: *phase_plus -> IFFT -> (m=0: real part) -> *profile(x)
 // -> FFT -> *phase_minus.  (Solver.damp_fields, solver.py:32-35.)
+// The damping round trip of one spectral row held in registers (element
+// x = threadIdx.x + q*FFT_NT in vv[q]); buf is the row's LDS buffer.  The
+// map is self-adjoint, so the adjoint kernel applies it unchanged.
+template <typename T, int N>
+__device__ __forceinline__ void damp_row(typename C2<T>::t *vv, typename C2<T>::t *buf, int m,
+                                         const T *__restrict__ damp, int nd,
+                                         const T *__restrict__ php, const T *__restrict__ phm,
+                                         const T *__restrict__ tw) {
+  typedef typename C2<T>::t T2;
+  constexpr int PER = (N + FFT_NT - 1) / FFT_NT;
+  const T inv = (T)1 / (T)N;
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int x = threadIdx.x + q * FFT_NT;
+    if (x < N) buf[x] = c_mul(vv[q], ((const T2 *)php)[x]);
+  }
+  __syncthreads();
+  fft_lds<T, N, 1, 1, FFT_NT>(buf, (const T2 *)tw);
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int x = threadIdx.x + q * FFT_NT;
+    if (x < N) {
+      T2 v = buf[x];
+      v.x *= inv;
+      v.y *= inv;
+      if (m == 0) v.y = 0;
+      if (x < nd) { v.x *= damp[x]; v.y *= damp[x]; }
+      if (x > N - nd) { v.x *= damp[N - x]; v.y *= damp[N - x]; }
+      buf[x] = v;
+    }
+  }
+  __syncthreads();
+  fft_lds<T, N, -1, 1, FFT_NT>(buf, (const T2 *)tw);
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int x = threadIdx.x + q * FFT_NT;
+    if (x < N) vv[q] = c_mul(buf[x], ((const T2 *)phm)[x]);
+  }
+  __syncthreads();
+}
+
 template <typename T, int N, bool ADV>
 __global__ void __launch_bounds__(FFT_NT) k_damp(SolvPlanes sv, const T *__restrict__ damp, int nd,
                                                  const T *__restrict__ php, const T *__restrict__ phm,
@@ -562,38 +603,15 @@ __global__ void __launch_bounds__(FFT_NT) k_damp(SolvPlanes sv, const T *__restr
       gv[q] = g0;
     }
   }
-  const T inv = (T)1 / (T)N;
-  for (int h = 0; h < 2; h++) {
-    T2 *vv = h == 0 ? ev : gv;
+  damp_row<T, N>(ev, buf, m, damp, nd, php, phm, tw);
+  damp_row<T, N>(gv, buf, m, damp, nd, php, phm, tw);
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-      int x = threadIdx.x + q * FFT_NT;
-      if (x < N) buf[x] = c_mul(vv[q], ((const T2 *)php)[x]);
+  for (int q = 0; q < PER; q++) {
+    int x = threadIdx.x + q * FFT_NT;
+    if (x < N) {
+      ((T2 *)sv.e[c][m])[rowoff + x] = ev[q];
+      ((T2 *)sv.g[c][m])[rowoff + x] = gv[q];
     }
-    __syncthreads();
-    fft_lds<T, N, 1, 1, FFT_NT>(buf, (const T2 *)tw);
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-      int x = threadIdx.x + q * FFT_NT;
-      if (x < N) {
-        T2 v = buf[x];
-        v.x *= inv;
-        v.y *= inv;
-        if (m == 0) v.y = 0;
-        if (x < nd) { v.x *= damp[x]; v.y *= damp[x]; }
-        if (x > N - nd) { v.x *= damp[N - x]; v.y *= damp[N - x]; }
-        buf[x] = v;
-      }
-    }
-    __syncthreads();
-    fft_lds<T, N, -1, 1, FFT_NT>(buf, (const T2 *)tw);
-    void *dst = h == 0 ? sv.e[c][m] : sv.g[c][m];
-#pragma unroll
-    for (int q = 0; q < PER; q++) {
-      int x = threadIdx.x + q * FFT_NT;
-      if (x < N) ((T2 *)dst)[rowoff + x] = c_mul(buf[x], ((const T2 *)phm)[x]);
-    }
-    __syncthreads();
   }
 }
 
@@ -637,6 +655,369 @@ extern "C" int chm_damp_fields_f32(const chm_spectral_t *sp, const chm_fb_t *E, 
                                    int32_t advance_first, const chm_fb_t *J, const chm_fb_t *dN0,
                                    const chm_fb_t *dN1, void *stream) {
   return damp_fields<float>(sp, E, G, advance_first, J, dN0, dN1, stream);
+}
+
+// ---------------------------------------------------------------- adjoints
+// Adjoints of the spectral operators for the differentiable form.  Complex
+// adjoints are dL/dRe + i dL/dIm of every stored element (the m = 0 planes'
+// entries are treated as independent).  The DHT-type products use a
+// chm_spectral_t whose dDp / dDm slots hold the TRANSPOSED tables
+// (chm_pad_table of dDHT_plus^T, dDHT_minus^T), so that one product
+// A^T (alpha X + beta Y) is a forward product with another table.
+//
+// For a forward product P = D (alpha X) read through the m = -1 map
+// (X -> -conj(X(-kx))), the adjoint is Xbar += D^T (alpha remap(Pbar));
+// without the map it is Xbar += D^T (conj(alpha) Pbar).
+
+struct AdjTerm {
+  const void *p;
+  double cr, ci;
+};
+struct AdjTarget {
+  void *out;
+  const void *kxsrc;  // optional: adds (kr + i ki) kx * kxsrc
+  double kr, ki;
+  int nt;
+  AdjTerm t[3];
+};
+#define ADJ_MAXT (3 * (CHM_MAXM + 1))
+struct AdjAssemble {
+  AdjTarget tg[ADJ_MAXT];
+  int acc;
+};
+
+template <typename T>
+__global__ void k_adj_assemble(AdjAssemble aa, const T *__restrict__ kx, int Nx, int64_t n) {
+  typedef typename C2<T>::t T2;
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const AdjTarget &tg = aa.tg[blockIdx.y];
+  T2 v;
+  v.x = 0;
+  v.y = 0;
+  if (aa.acc) v = ((const T2 *)tg.out)[e];
+  for (int i = 0; i < tg.nt; i++) {
+    T2 s = ((const T2 *)tg.t[i].p)[e];
+    T cr = (T)tg.t[i].cr, ci = (T)tg.t[i].ci;
+    v.x += cr * s.x - ci * s.y;
+    v.y += cr * s.y + ci * s.x;
+  }
+  if (tg.kxsrc) {
+    T2 s = ((const T2 *)tg.kxsrc)[e];
+    T k = kx[e % Nx];
+    T cr = (T)tg.kr * k, ci = (T)tg.ki * k;
+    v.x += cr * s.x - ci * s.y;
+    v.y += cr * s.y + ci * s.x;
+  }
+  ((T2 *)tg.out)[e] = v;
+}
+
+// OP 0: grad^H  (in = adjoint of (x, y, z) of grad, out = adjoint of the scalar)
+// OP 2: rot^H   (in = adjoint of rot's x, y, z; out = adjoint of its input;
+//                with poisson, in is first multiplied by 1/w^2 in place)
+template <typename T>
+static int spectral_op_adj(int OP, const chm_spectral_t *sp, const chm_fb_t *in,
+                           const chm_fb_t *out, int32_t poisson, int32_t acc, void *ws,
+                           size_t ws_bytes, void *stream) {
+  if (!sp || !in || !out || sp->M < 0 || sp->M > CHM_MAXM) return CHM_EINVAL;
+  const int Nx = sp->Nx, Nk = sp->Nr - 1, M = sp->M;
+  if (chm_spectral_workspace_bytes(Nx, sp->Nr, M, sizeof(T)) > ws_bytes || !ws) return CHM_EWORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = (int64_t)Nk * Nx;
+  int rc = CHM_OK;
+  if (OP == 2 && poisson) {
+    TabPlanes tp = {};
+    int np = 0;
+    for (int m = 0; m <= M; m++)
+      for (int c = 0; c < 3; c++) {
+        tp.p[np] = in[c].m[m];
+        tp.t[np++] = sp->c3[m];
+      }
+    k_mult_table<T><<<dim3(chm_blocks(n, 256), np), 256, 0, st>>>(tp, n);
+  }
+  const size_t pe = (size_t)Nk * Nx * 2;
+  T *scr = (T *)ws;
+  GemmBatch gb = {};
+  gb.M = Nk;
+  gb.K = Nk;
+  gb.N = 2 * Nx;
+  gb.Nx = Nx;
+  gb.lda = gemm_lda(Nk);
+  gb.ldc = 2 * Nx;
+  gb.bmode = 1;
+  int ni = 0, ns = 0;
+  const int kind = gemm_kind(sp, sizeof(T));
+  auto item = [&](const void *A, const void *Alo, const void *X, double2 a, int rx, const void *Y,
+                  double2 b, int ry) -> const void * {
+    if (ni == GEMM_MAXB) {
+      if ((rc = chm_gemm_launch(gb, ni, kind, st))) return nullptr;
+      ni = 0;
+    }
+    GemmItem &it = gb.it[ni++];
+    it.A = A;
+    it.A_lo = Alo;
+    it.X = X;
+    it.Y = Y;
+    it.alpha = a;
+    it.beta = b;
+    it.remapX = rx;
+    it.remapY = ry;
+    it.C = scr + (size_t)(ns++) * pe;
+    return it.C;
+  };
+  const double2 one = {1, 0}, mone = {-1, 0}, I = {0, 1}, mI = {0, -1}, zero = {0, 0};
+  AdjAssemble aa = {};
+  aa.acc = acc;
+  const int ncomp_out = OP == 0 ? 1 : 3;
+  for (int s = 0; s <= M; s++)
+    for (int c = 0; c < ncomp_out; c++) aa.tg[s * ncomp_out + c].out = out[c].m[s];
+  auto add = [&](int s, int c, const void *p, double cr, double ci) {
+    AdjTarget &t = aa.tg[s * ncomp_out + c];
+    t.t[t.nt].p = p;
+    t.t[t.nt].cr = cr;
+    t.t[t.nt].ci = ci;
+    t.nt++;
+  };
+  for (int m = 0; m <= M; m++) {
+    const bool lower = m > 0 || M > 0;
+    if (OP == 0) {
+      // grad: y = -P0 + P1, z = -i P0 - i P1, P0 = D-_m S_{m-1}, P1 = D+_m S_{m+1}
+      if (lower) {
+        const void *q;
+        if (m > 0) q = item(sp->dDm[m], sp->dDm_lo[m], in[1].m[m], mone, 0, in[2].m[m], I, 0);
+        else q = item(sp->dDm[0], sp->dDm_lo[0], in[1].m[0], mone, 1, in[2].m[0], mI, 1);
+        if (rc) return rc;
+        add(m > 0 ? m - 1 : 1, 0, q, 1, 0);
+      }
+      if (m < M) {
+        const void *q = item(sp->dDp[m], sp->dDp_lo[m], in[1].m[m], one, 0, in[2].m[m], I, 0);
+        if (rc) return rc;
+        add(m + 1, 0, q, 1, 0);
+      }
+    } else {
+      // rot: bx = P0 + P2, by = -i (P1 + P3), bz = P1 - P3 (plus kx terms),
+      // P0 = D-(-Gz + i Gy)_{m-1}, P1 = D- Gx_{m-1}, P2 = D+(Gz + i Gy)_{m+1}, P3 = D+ Gx_{m+1}
+      if (lower) {
+        if (m > 0) {
+          const void *qa = item(sp->dDm[m], sp->dDm_lo[m], in[0].m[m], one, 0, nullptr, zero, 0);
+          const void *qb = item(sp->dDm[m], sp->dDm_lo[m], in[1].m[m], I, 0, in[2].m[m], one, 0);
+          if (rc) return rc;
+          add(m - 1, 2, qa, -1, 0);
+          add(m - 1, 1, qa, 0, -1);
+          add(m - 1, 0, qb, 1, 0);
+        } else {
+          const void *qa = item(sp->dDm[0], sp->dDm_lo[0], in[0].m[0], one, 1, nullptr, zero, 0);
+          const void *qb = item(sp->dDm[0], sp->dDm_lo[0], in[1].m[0], mI, 1, in[2].m[0], one, 1);
+          if (rc) return rc;
+          add(1, 2, qa, -1, 0);
+          add(1, 1, qa, 0, 1);
+          add(1, 0, qb, 1, 0);
+        }
+      }
+      if (m < M) {
+        const void *qc = item(sp->dDp[m], sp->dDp_lo[m], in[0].m[m], one, 0, nullptr, zero, 0);
+        const void *qd = item(sp->dDp[m], sp->dDp_lo[m], in[1].m[m], I, 0, in[2].m[m], mone, 0);
+        if (rc) return rc;
+        add(m + 1, 2, qc, 1, 0);
+        add(m + 1, 1, qc, 0, -1);
+        add(m + 1, 0, qd, 1, 0);
+      }
+    }
+  }
+  if (ni > 0 && (rc = chm_gemm_launch(gb, ni, kind, st))) return rc;
+  for (int s = 0; s <= M; s++) {
+    if (OP == 0) {
+      AdjTarget &t = aa.tg[s];
+      t.kxsrc = in[0].m[s];
+      t.kr = 0;
+      t.ki = -1;
+    } else {
+      AdjTarget &tz = aa.tg[s * 3 + 2], &ty = aa.tg[s * 3 + 1];
+      tz.kxsrc = in[1].m[s];
+      tz.kr = 0;
+      tz.ki = 1;
+      ty.kxsrc = in[2].m[s];
+      ty.kr = 0;
+      ty.ki = -1;
+    }
+  }
+  dim3 g(chm_blocks(n, 256), (M + 1) * ncomp_out);
+  k_adj_assemble<T><<<g, 256, 0, st>>>(aa, (const T *)sp->kx, Nx, n);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_field_grad_adj_f64(const chm_spectral_t *sp, const chm_fb_t *vec_b,
+                                      const chm_fb_t *scl_b, int32_t accumulate, void *ws,
+                                      size_t ws_bytes, void *stream) {
+  return spectral_op_adj<double>(0, sp, vec_b, scl_b, 0, accumulate, ws, ws_bytes, stream);
+}
+extern "C" int chm_field_grad_adj_f32(const chm_spectral_t *sp, const chm_fb_t *vec_b,
+                                      const chm_fb_t *scl_b, int32_t accumulate, void *ws,
+                                      size_t ws_bytes, void *stream) {
+  return spectral_op_adj<float>(0, sp, vec_b, scl_b, 0, accumulate, ws, ws_bytes, stream);
+}
+extern "C" int chm_field_rot_adj_f64(const chm_spectral_t *sp, const chm_fb_t *in_b,
+                                     const chm_fb_t *out_b, int32_t poisson, int32_t accumulate,
+                                     void *ws, size_t ws_bytes, void *stream) {
+  return spectral_op_adj<double>(2, sp, in_b, out_b, poisson, accumulate, ws, ws_bytes, stream);
+}
+extern "C" int chm_field_rot_adj_f32(const chm_spectral_t *sp, const chm_fb_t *in_b,
+                                     const chm_fb_t *out_b, int32_t poisson, int32_t accumulate,
+                                     void *ws, size_t ws_bytes, void *stream) {
+  return spectral_op_adj<float>(2, sp, in_b, out_b, poisson, accumulate, ws, ws_bytes, stream);
+}
+
+// Transpose of the PSATD update of one element (real coefficients):
+// (E', G') = L (E, G, J, n0, n1) -> (Ebar, Gbar, Jbar, n0bar, n1bar) = L^T (E'bar, G'bar).
+template <typename T>
+__device__ __forceinline__ void psatd_adj(T c1, T c2, T c3, T dt_inv, T eb1, T gb1, T &eb0, T &gb0,
+                                          T &jb, T &n0b, T &n1b) {
+  const T pi2 = (T)(2 * 3.14159265358979323846);
+  const T c23 = c2 * c3;
+  eb0 = c1 * eb1 - c2 * gb1;
+  gb0 = c23 * eb1 + c1 * gb1;
+  jb = pi2 * (-c23 * eb1 + ((T)1 - c1) * gb1);
+  n0b = pi2 * (c3 * (c1 - dt_inv * c23) * eb1 + c3 * (dt_inv * ((T)1 - c1) - c2) * gb1);
+  n1b = pi2 * (c3 * (dt_inv * c23 - (T)1) * eb1 - c3 * dt_inv * ((T)1 - c1) * gb1);
+}
+
+struct SolvAdj {
+  void *e[3][CHM_MAXM + 1], *g[3][CHM_MAXM + 1];
+  const void *eadd[3][CHM_MAXM + 1];
+  void *j[3][CHM_MAXM + 1], *carry[3][CHM_MAXM + 1], *n1[3][CHM_MAXM + 1];
+  const void *c1[CHM_MAXM + 1], *c2[CHM_MAXM + 1], *c3[CHM_MAXM + 1];
+};
+
+template <typename T>
+__device__ __forceinline__ void psatd_adj_store(const SolvAdj &sa, int c, int m, int64_t id,
+                                                typename C2<T>::t eb, typename C2<T>::t gb,
+                                                T dt_inv) {
+  typedef typename C2<T>::t T2;
+  T c1 = ((const T *)sa.c1[m])[id], c2 = ((const T *)sa.c2[m])[id], c3 = ((const T *)sa.c3[m])[id];
+  T2 e0, g0, jb, a, b;
+  psatd_adj<T>(c1, c2, c3, dt_inv, eb.x, gb.x, e0.x, g0.x, jb.x, a.x, b.x);
+  psatd_adj<T>(c1, c2, c3, dt_inv, eb.y, gb.y, e0.y, g0.y, jb.y, a.y, b.y);
+  T2 cin = ((T2 *)sa.carry[c][m])[id];
+  b.x += cin.x;
+  b.y += cin.y;
+  ((T2 *)sa.e[c][m])[id] = e0;
+  ((T2 *)sa.g[c][m])[id] = g0;
+  ((T2 *)sa.j[c][m])[id] = jb;
+  ((T2 *)sa.n1[c][m])[id] = b;
+  ((T2 *)sa.carry[c][m])[id] = a;
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(FFT_NT) k_damp_adj(SolvAdj sa, const T *__restrict__ damp, int nd,
+                                                     const T *__restrict__ php,
+                                                     const T *__restrict__ phm,
+                                                     const T *__restrict__ tw, T dt_inv) {
+  typedef typename C2<T>::t T2;
+  constexpr int PER = (N + FFT_NT - 1) / FFT_NT;
+  __shared__ T2 buf[N];
+  const int kr = blockIdx.x, m = blockIdx.y, c = blockIdx.z;
+  const int64_t rowoff = (int64_t)kr * N;
+  T2 ev[PER], gv[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int x = threadIdx.x + q * FFT_NT;
+    if (x < N) {
+      T2 e = ((T2 *)sa.e[c][m])[rowoff + x];
+      if (sa.eadd[c][m]) {
+        T2 a = ((const T2 *)sa.eadd[c][m])[rowoff + x];
+        e.x += a.x;
+        e.y += a.y;
+      }
+      ev[q] = e;
+      gv[q] = ((T2 *)sa.g[c][m])[rowoff + x];
+    }
+  }
+  damp_row<T, N>(ev, buf, m, damp, nd, php, phm, tw);
+  damp_row<T, N>(gv, buf, m, damp, nd, php, phm, tw);
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int x = threadIdx.x + q * FFT_NT;
+    if (x < N) psatd_adj_store<T>(sa, c, m, rowoff + x, ev[q], gv[q], dt_inv);
+  }
+}
+
+template <typename T> __global__ void k_advance_adj(SolvAdj sa, T dt_inv, int64_t n) {
+  typedef typename C2<T>::t T2;
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int m = blockIdx.y, c = blockIdx.z;
+  if (e >= n) return;
+  T2 eb = ((T2 *)sa.e[c][m])[e];
+  if (sa.eadd[c][m]) {
+    T2 a = ((const T2 *)sa.eadd[c][m])[e];
+    eb.x += a.x;
+    eb.y += a.y;
+  }
+  T2 gb = ((T2 *)sa.g[c][m])[e];
+  psatd_adj_store<T>(sa, c, m, e, eb, gb, dt_inv);
+}
+
+// Adjoint of damp_fields with the PSATD advance first (the loop's
+// push_fields + damp_fields).  In: E_b, G_b = adjoints of the new E, G
+// (plus E_add, if given, added to E_b), carry = adjoint of the dN1 this
+// step produces as seen by the NEXT step's advance (as its dN0).  Out:
+// E_b, G_b = adjoints of the old E, G; J_b; N1_b = total adjoint of this
+// step's dN1; carry = adjoint of this step's dN0 (for the previous step).
+template <typename T>
+static int damp_fields_adj(const chm_spectral_t *sp, const chm_fb_t *E, const chm_fb_t *G,
+                           const chm_fb_t *Eadd, const chm_fb_t *J, const chm_fb_t *carry,
+                           const chm_fb_t *N1, void *stream) {
+  if (!sp || !E || !G || !J || !carry || !N1 || sp->M < 0 || sp->M > CHM_MAXM) return CHM_EINVAL;
+  SolvAdj sa = {};
+  for (int m = 0; m <= sp->M; m++) {
+    for (int k = 0; k < 3; k++) {
+      sa.e[k][m] = E[k].m[m];
+      sa.g[k][m] = G[k].m[m];
+      sa.eadd[k][m] = Eadd ? Eadd[k].m[m] : nullptr;
+      sa.j[k][m] = J[k].m[m];
+      sa.carry[k][m] = carry[k].m[m];
+      sa.n1[k][m] = N1[k].m[m];
+    }
+    sa.c1[m] = sp->c1[m];
+    sa.c2[m] = sp->c2[m];
+    sa.c3[m] = sp->c3[m];
+  }
+  hipStream_t st = (hipStream_t)stream;
+  T dti = (T)sp->dt_inv;
+  if (sp->ndamp <= 0 || !sp->damp) {
+    int64_t n = (int64_t)(sp->Nr - 1) * sp->Nx;
+    k_advance_adj<T><<<dim3(chm_blocks(n, 256), sp->M + 1, 3), 256, 0, st>>>(sa, dti, n);
+    CHM_CHECK_LAUNCH();
+    return CHM_OK;
+  }
+  const int Nx = sp->Nx;
+  if (!fft_size_ok(Nx)) return CHM_EUNSUPPORTED;
+  dim3 g(sp->Nr - 1, sp->M + 1, 3);
+  const T *dp = (const T *)sp->damp, *php = (const T *)sp->phase_plus, *phm = (const T *)sp->phase_minus,
+          *tw = (const T *)sp->twiddle;
+  switch (Nx) {
+#define CASE(NN)                                                                   \
+  case NN:                                                                         \
+    k_damp_adj<T, NN><<<g, FFT_NT, 0, st>>>(sa, dp, sp->ndamp, php, phm, tw, dti); \
+    break;
+    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096)
+#undef CASE
+  }
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_damp_fields_adj_f64(const chm_spectral_t *sp, const chm_fb_t *E_b,
+                                       const chm_fb_t *G_b, const chm_fb_t *E_add,
+                                       const chm_fb_t *J_b, const chm_fb_t *carry,
+                                       const chm_fb_t *N1_b, void *stream) {
+  return damp_fields_adj<double>(sp, E_b, G_b, E_add, J_b, carry, N1_b, stream);
+}
+extern "C" int chm_damp_fields_adj_f32(const chm_spectral_t *sp, const chm_fb_t *E_b,
+                                       const chm_fb_t *G_b, const chm_fb_t *E_add,
+                                       const chm_fb_t *J_b, const chm_fb_t *carry,
+                                       const chm_fb_t *N1_b, void *stream) {
+  return damp_fields_adj<float>(sp, E_b, G_b, E_add, J_b, carry, N1_b, stream);
 }
 
 // ---------------------------------------------------------------- laser
@@ -697,6 +1078,32 @@ extern "C" int chm_laser_spectral_f64(const chm_spectral_t *sp, const double *w_
   int64_t n = (int64_t)(sp->Nr - 1) * sp->Nx;
   k_laser_spectral<<<chm_blocks(n, 256), 256, 0, (hipStream_t)stream>>>(
       (const double *)sp->kx, w_m0, (double2 *)Ez_fb_m0, (double2 *)Gz_fb_m0, X_focus, sp->Nx, n);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+// d/dR of the pulse above (R = waist): Ez * 2 r^2 / R^3 inside the cut-offs.
+__global__ void k_laser_pulse_dR(double *Ez, int Nx, int Nr, double xmin, double dx, double rmin,
+                                 double dr, double k0, double a0, double x0, double Lx, double R) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)(Nr - 1) * Nx) return;
+  int ir = 1 + (int)(e / Nx), ix = (int)(e % Nx);
+  double x = xmin + dx * (double)ix;
+  double r = rmin + dr * (double)ir;
+  double xs = x - x0;
+  double v = a0 * sin(k0 * xs) * exp(-xs * xs / (Lx * Lx) - r * r / (R * R));
+  v = v * (2.0 * r * r / (R * R * R));
+  if (!(fabs(r) < 3.5 * R) || !(fabs(xs) < 3.5 * Lx)) v = 0.0 * v;
+  Ez[(int64_t)ir * Nx + ix] = v;
+}
+
+extern "C" int chm_laser_pulse_dR_f64(double *Ez_m0, int32_t Nx, int32_t Nr, double xmin,
+                                      double dx, double rmin, double dr, double k0, double a0,
+                                      double x0, double Lx, double R, void *stream) {
+  if (!Ez_m0 || Nx < 1 || Nr < 2) return CHM_EINVAL;
+  int64_t n = (int64_t)(Nr - 1) * Nx;
+  k_laser_pulse_dR<<<chm_blocks(n, 256), 256, 0, (hipStream_t)stream>>>(Ez_m0, Nx, Nr, xmin, dx,
+                                                                         rmin, dr, k0, a0, x0, Lx, R);
   CHM_CHECK_LAUNCH();
   return CHM_OK;
 }

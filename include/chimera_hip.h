@@ -382,6 +382,167 @@ int chm_laser_spectral_f64(const chm_spectral_t *sp, const double *w_m0,
                            void *Ez_fb_m0, void *Gz_fb_m0, double X_focus,
                            void *stream);
 
+/* ------------------------------------------------------------------ adjoints
+ * Reverse-mode kernels of the differentiable form (no counterpart in the
+ * reference, whose step is not differentiable).  Adjoints of real values
+ * are dL/dv; of complex values dL/dRe + i dL/dIm, every stored element
+ * independent.  Particle adjoints are SoA arrays indexed like the
+ * particles, and every entry point ADDS into them, except where stated.
+ * Each is the exact transpose (vector-Jacobian product) of the forward
+ * entry point named, at the forward inputs given. */
+
+/* push_x (chm_push_x_*): x' = x + u g_inv dt.  u_b += x_b g_inv dt,
+ * g_b += dt (x_b . u); x_b passes through unchanged. */
+int chm_push_x_adj_f64(const double *xb, const double *yb, const double *zb,
+                       const double *ux, const double *uy, const double *uz,
+                       const double *g_inv, double *uxb, double *uyb,
+                       double *uzb, double *gb, double dt, int64_t Np,
+                       void *stream);
+int chm_push_x_adj_f32(const float *xb, const float *yb, const float *zb,
+                       const float *ux, const float *uy, const float *uz,
+                       const float *g_inv, float *uxb, float *uyb, float *uzb,
+                       float *gb, float dt, int64_t Np, void *stream);
+
+/* deposit (chm_deposit_scalar_*, one species) given the adjoint planes of
+ * rho after chm_postproc_deposit_adj_*: adds to x_b, y_b, z_b, w_b. */
+int chm_deposit_scalar_adj_f64(const double *x, const double *y,
+                               const double *z, const double *w,
+                               int32_t charge, int64_t Np,
+                               const chm_grid_t *g, const chm_planes_t *rho_b,
+                               double *xb, double *yb, double *zb, double *wb,
+                               void *stream);
+int chm_deposit_scalar_adj_f32(const float *x, const float *y, const float *z,
+                               const float *w, int32_t charge, int64_t Np,
+                               const chm_grid_t *g, const chm_planes_t *rho_b,
+                               float *xb, float *yb, float *zb, float *wb,
+                               void *stream);
+/* deposit (chm_deposit_vector_*) given J_b[3]: adds to x_b, y_b, z_b,
+ * u_b (3), g_b, w_b. */
+int chm_deposit_vector_adj_f64(const double *x, const double *y,
+                               const double *z, const double *ux,
+                               const double *uy, const double *uz,
+                               const double *g_inv, const double *w,
+                               int32_t charge, int64_t Np, const chm_grid_t *g,
+                               const chm_planes_t *J_b, double *xb, double *yb,
+                               double *zb, double *uxb, double *uyb,
+                               double *uzb, double *gb, double *wb,
+                               void *stream);
+int chm_deposit_vector_adj_f32(const float *x, const float *y, const float *z,
+                               const float *ux, const float *uy,
+                               const float *uz, const float *g_inv,
+                               const float *w, int32_t charge, int64_t Np,
+                               const chm_grid_t *g, const chm_planes_t *J_b,
+                               float *xb, float *yb, float *zb, float *uxb,
+                               float *uyb, float *uzb, float *gb, float *wb,
+                               void *stream);
+
+/* gather + Boris push (chm_gather_push_*) at positions x, y, z and the
+ * momenta BEFORE the push, with the fields E[3], B[3] of that push.
+ * In: u_b, g_b = adjoints of the pushed momenta and g_inv.  Out (replaced):
+ * u_b = adjoint of the momenta before the push, g_b = 0 for pushed
+ * particles; added: x_b, y_b, z_b; written: fb[0..5] = per-particle
+ * adjoints of the gathered E_p, B_p (0 for particles not pushed), to be
+ * scattered with chm_gather_adj_scatter_*. */
+int chm_gather_push_adj_f64(const double *x, const double *y, const double *z,
+                            const double *ux, const double *uy,
+                            const double *uz, int64_t Np,
+                            const uint32_t *np_stay, double factor_push,
+                            const chm_grid_t *g, const chm_planes_t *E,
+                            const chm_planes_t *B, double *uxb, double *uyb,
+                            double *uzb, double *gb, double *xb, double *yb,
+                            double *zb, double *const *fb, void *stream);
+int chm_gather_push_adj_f32(const float *x, const float *y, const float *z,
+                            const float *ux, const float *uy, const float *uz,
+                            int64_t Np, const uint32_t *np_stay,
+                            float factor_push, const chm_grid_t *g,
+                            const chm_planes_t *E, const chm_planes_t *B,
+                            float *uxb, float *uyb, float *uzb, float *gb,
+                            float *xb, float *yb, float *zb,
+                            float *const *fb, void *stream);
+/* out[k] += sum_p C(p) v_k[p] (mode 0) and C(p) v_k[p] 2 ((y+iz)/r)^m
+ * (modes m >= 1): the transpose of the gather's interpolation, with the
+ * sorted order of the positions (sort_indx, cell_offset) and the vector
+ * deposit's two-pass scheme and workspace. */
+int chm_gather_adj_scatter_f64(const uint32_t *sort_indx,
+                               const uint32_t *cell_offset, const double *x,
+                               const double *y, const double *z,
+                               const double *v0, const double *v1,
+                               const double *v2, const chm_grid_t *g,
+                               const chm_planes_t *out, void *ws,
+                               size_t ws_bytes, void *stream);
+int chm_gather_adj_scatter_f32(const uint32_t *sort_indx,
+                               const uint32_t *cell_offset, const float *x,
+                               const float *y, const float *z, const float *v0,
+                               const float *v1, const float *v2,
+                               const chm_grid_t *g, const chm_planes_t *out,
+                               void *ws, size_t ws_bytes, void *stream);
+
+/* Transposes of chm_postproc_deposit_* and chm_warp_axis_*, in place. */
+int chm_postproc_deposit_adj_f64(const chm_planes_t *flds, int32_t nflds,
+                                 const double *dV_inv, int32_t Nx, int32_t Nr,
+                                 int32_t M, void *stream);
+int chm_postproc_deposit_adj_f32(const chm_planes_t *flds, int32_t nflds,
+                                 const double *dV_inv, int32_t Nx, int32_t Nr,
+                                 int32_t M, void *stream);
+int chm_warp_axis_adj_f64(const chm_planes_t *flds, int32_t nflds, int32_t Nx,
+                          int32_t Nr, int32_t M, void *stream);
+int chm_warp_axis_adj_f32(const chm_planes_t *flds, int32_t nflds, int32_t Nx,
+                          int32_t Nr, int32_t M, void *stream);
+
+/* Density profile (chm_dens_profile_*): knot_grad[j] += sum_p w_b[p]
+ * w_base[p] df(x_p)/dff_j, with x and w_base the positions and weights
+ * the profile was applied to; at most 128 knots; deterministic. */
+size_t chm_profile_adj_workspace_bytes(int32_t nknots);
+int chm_dens_profile_adj_f64(const double *x, const double *w_base,
+                             const double *w_bar, int64_t Np,
+                             const double *xx, const double *dxm1,
+                             int32_t nknots, double *knot_grad, void *ws,
+                             size_t ws_bytes, void *stream);
+int chm_dens_profile_adj_f32(const float *x, const float *w_base,
+                             const float *w_bar, int64_t Np, const double *xx,
+                             const double *dxm1, int32_t nknots,
+                             double *knot_grad, void *ws, size_t ws_bytes,
+                             void *stream);
+
+/* Spectral operators.  sp must hold the TRANSPOSED dDHT tables in dDp/dDm
+ * (and the transposed-and-scaled DHT tables, Nx DHT^T in DHT_inv and
+ * DHT_inv^T / Nx in DHT, for chm_transform_* to act as the adjoint of the
+ * opposite transform).  accumulate = 1 adds into the outputs. */
+int chm_field_grad_adj_f64(const chm_spectral_t *sp, const chm_fb_t *vec_b,
+                           const chm_fb_t *scl_b, int32_t accumulate, void *ws,
+                           size_t ws_bytes, void *stream);
+int chm_field_grad_adj_f32(const chm_spectral_t *sp, const chm_fb_t *vec_b,
+                           const chm_fb_t *scl_b, int32_t accumulate, void *ws,
+                           size_t ws_bytes, void *stream);
+/* poisson = 1: the adjoint of rot followed by *1/w^2; in_b is multiplied by
+ * 1/w^2 in place first. */
+int chm_field_rot_adj_f64(const chm_spectral_t *sp, const chm_fb_t *in_b,
+                          const chm_fb_t *out_b, int32_t poisson,
+                          int32_t accumulate, void *ws, size_t ws_bytes,
+                          void *stream);
+int chm_field_rot_adj_f32(const chm_spectral_t *sp, const chm_fb_t *in_b,
+                          const chm_fb_t *out_b, int32_t poisson,
+                          int32_t accumulate, void *ws, size_t ws_bytes,
+                          void *stream);
+/* Transpose of chm_damp_fields_* with advance = 1 (PSATD, then damping):
+ * in E_b, G_b (+ E_add if not NULL) = adjoints of the new E, G; carry =
+ * adjoint of this step's dN1 as the next step's dN0.  Out: E_b, G_b of the
+ * old E, G; J_b; N1_b = total adjoint of dN1; carry = adjoint of dN0.
+ * Uses the forward tables (sp as for chm_damp_fields_*). */
+int chm_damp_fields_adj_f64(const chm_spectral_t *sp, const chm_fb_t *E_b,
+                            const chm_fb_t *G_b, const chm_fb_t *E_add,
+                            const chm_fb_t *J_b, const chm_fb_t *carry,
+                            const chm_fb_t *N1_b, void *stream);
+int chm_damp_fields_adj_f32(const chm_spectral_t *sp, const chm_fb_t *E_b,
+                            const chm_fb_t *G_b, const chm_fb_t *E_add,
+                            const chm_fb_t *J_b, const chm_fb_t *carry,
+                            const chm_fb_t *N1_b, void *stream);
+/* d/dR of chm_laser_pulse_f64 (R = waist), same layout and cut-offs. */
+int chm_laser_pulse_dR_f64(double *Ez_m0, int32_t Nx, int32_t Nr, double xmin,
+                           double dx, double rmin, double dr, double k0,
+                           double a0, double x0, double Lx, double R,
+                           void *stream);
+
 #ifdef __cplusplus
 }
 #endif
