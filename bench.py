@@ -1,24 +1,33 @@
 """Benchmark of the chimeraCL PIC step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C2]
-                    [--precision f64|mixed] [--no-cpu]
+                    [--no-cpu] [--no-mixed] [--no-diff]
 
 Workload (default): the medium laser-wakefield configuration, Nz=2048,
 Nr=256, modes m=0,1, 2x2x8 particles per cell (16.6M electrons plus as many
-immobile ions), dt=dz, seeded synthetic plasma and a0=2 Gaussian laser.  A
-step is one full PIC step (push, sort, J and rho deposit, transforms,
+immobile ions), dt=dz, seeded synthetic plasma and an a0=2 Gaussian laser.
+A step is one full PIC step (push, sort, J and rho deposit, transforms,
 grad, PSATD + damping, B restore, backward transforms, gather + push); the
 timed steps replay a HIP graph of the step with every input resident in HBM.
 
-Printed: one JSON line with particle-steps/s (value), PIC steps/s, the
-relative L2 error of the GPU state against the CPU FP64 oracle after the
-first steps, the roofline figures of the dominant kernel measured with HIP
-events, and the CPU oracle's throughput on a bounded sample of the same
-workload (timed on this host's cores).
+The same workload runs twice, in the plain FP64 form (value) and in the
+mixed form (FP32 particles and FFT, split-BF16 3-term DHT products); the
+two runs advance in lockstep so that the mixed form's E, B, J and momentum
+errors and both charge-conservation residuals are measured against the FP64
+run at steps 100 and 500.  The FP64 run's error against the CPU FP64 oracle
+is measured after the first steps, with the oracle's throughput on that
+bounded sample timed on this host's cores.  The differentiable form is
+measured on the two differentiable workloads (Nz=1024, Nr=128, 100 steps,
+FP64, gradients against central finite differences; Nz=4096, Nr=256, 400
+steps, mixed, gradients against the FP64 adjoint), with the backward /
+forward time ratio.
+
+Printed: one JSON line.
 """
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -30,6 +39,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
 
 
 def parse():
@@ -38,10 +48,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="C2")
-    ap.add_argument("--precision", default="f64", choices=["f64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--diag-steps", default="100,500")
     return ap.parse_args()
 
 
@@ -49,69 +61,320 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_sim(cfg_name, precision, with_oracle):
-    import chimeracl_amd as CA
-    from chimeracl_amd import config as C
-    from chimeracl_amd.methods.generic_methods_hip import Communicator
-    comm = Communicator(precision=precision)
-    g = C.grid_config(cfg_name)
-    laser = C.laser_config(cfg_name)
-    c = C.CONFIGS[cfg_name]
-    solver = CA.Solver(dict(g), comm)
-    CA.add_gausian_pulse(solver, dict(laser))
-    eo, io = C.species_config(cfg_name, solver.Args)
-    t = time.time()
-    arr = C.load_plasma(g, c["ppc"], c["dens"], seed=0)
-    log(f"[bench] loaded {arr['x'].size} particles in {time.time() - t:.1f}s")
-    eons, ions = CA.Particles(dict(eo), comm), CA.Particles(dict(io), comm)
-    eons.add_particles(source=C.ParticleSource(arr))
-    ions.add_particles(source=C.ParticleSource(arr))
-    prof = C.density_profile(cfg_name, g)
-    if prof is not None:
-        eons.dens_profile(*prof)
-        ions.dens_profile(*prof)
-    loop = CA.PIC_loop(solvers=[solver], species=[eons, ions], graph=True)
-    orc = None
-    if with_oracle:
-        from oracle import oracle as O
-        s = O.OracleSolver(dict(g))
-        O.add_gausian_pulse(s, dict(laser))
-        e, i = O.OracleParticles(dict(eo)), O.OracleParticles(dict(io))
-        e.add_particles(arr)
-        i.add_particles(arr)
+class Sim:
+    def __init__(self, cfg_name, precision, with_oracle):
+        import chimeracl_amd as CA
+        from chimeracl_amd import config as C
+        from chimeracl_amd.methods.generic_methods_hip import Communicator
+        self.comm = Communicator(precision=precision)
+        g = self.g = C.grid_config(cfg_name)
+        laser = C.laser_config(cfg_name)
+        c = C.CONFIGS[cfg_name]
+        self.solver = CA.Solver(dict(g), self.comm)
+        CA.add_gausian_pulse(self.solver, dict(laser))
+        eo, io = C.species_config(cfg_name, self.solver.Args)
+        t = time.time()
+        arr = C.load_plasma(g, c["ppc"], c["dens"], seed=0)
+        log(f"[bench] {precision}: loaded {arr['x'].size} particles in {time.time() - t:.1f}s")
+        self.eons, self.ions = CA.Particles(dict(eo), self.comm), CA.Particles(dict(io), self.comm)
+        self.eons.add_particles(source=C.ParticleSource(arr))
+        self.ions.add_particles(source=C.ParticleSource(arr))
+        prof = C.density_profile(cfg_name, g)
         if prof is not None:
-            e.dens_profile(*prof)
-            i.dens_profile(*prof)
-        orc = (s, e, i, O.OracleLoop(s, [e, i]))
-    return g, solver, eons, ions, loop, orc
+            self.eons.dens_profile(*prof)
+            self.ions.dens_profile(*prof)
+        self.loop = CA.PIC_loop(solvers=[self.solver], species=[self.eons, self.ions], graph=True)
+        self.steps = 0
+        self.orc = None
+        if with_oracle:
+            from oracle import oracle as O
+            s = O.OracleSolver(dict(g))
+            O.add_gausian_pulse(s, dict(laser))
+            e, i = O.OracleParticles(dict(eo)), O.OracleParticles(dict(io))
+            e.add_particles(arr)
+            i.add_particles(arr)
+            if prof is not None:
+                e.dens_profile(*prof)
+                i.dens_profile(*prof)
+            self.orc = (s, e, i, O.OracleLoop(s, [e, i]))
+
+    def step(self, n=1):
+        for _ in range(n):
+            self.loop.step()
+        self.steps += n
 
 
-def rel_l2(a, b):
-    n = np.linalg.norm(b.ravel())
-    return float(np.linalg.norm((a - b).ravel()) / n) if n > 0 else float(np.linalg.norm(a.ravel()))
-
-
-def oracle_errors(solver, eons, orc, M):
-    s, e = orc[0], orc[1]
+def oracle_errors(sim, M):
+    s, e = sim.orc[0], sim.orc[1]
     out = {}
     for f in ("E", "B", "J"):
         num = den = 0.0
         for c in "xyz":
             for m in range(M + 1):
-                a = solver.DataDev[f"{f}{c}_m{m}"].detach().cpu().numpy()[1:]
+                a = sim.solver.DataDev[f"{f}{c}_m{m}"].detach().cpu().numpy()[1:]
                 b = s.f[f"{f}{c}_m{m}"][1:]
                 num += float(np.linalg.norm((a - b).ravel()) ** 2)
                 den += float(np.linalg.norm(b.ravel()) ** 2)
         out[f] = (num / den) ** 0.5 if den > 0 else num ** 0.5
     num = den = 0.0
     for k in ("px", "py", "pz"):
-        a, b = eons.host(k), e.data[k]
+        a, b = sim.eons.host(k), e.data[k]
         num += float(np.linalg.norm(a - b) ** 2)
         den += float(np.linalg.norm(b) ** 2)
     out["p"] = (num / den) ** 0.5
     return out
 
 
+# ---------------------------------------------------------------- diagnostics (FP64)
+def _rel(pairs):
+    num = den = 0.0
+    for a, b in pairs:
+        d = (a.to(b.dtype) - b)
+        num += float((d.abs() ** 2).sum())
+        den += float((b.abs() ** 2).sum())
+    return (num / den) ** 0.5 if den > 0 else num ** 0.5
+
+
+def mixed_errors(mx, fp, M):
+    """Relative L2 of the mixed run against the FP64 run: real-space E, B, J
+    (rows 1..Nr-1, every component and mode) and the electron momenta."""
+    import torch
+    out = {}
+    for f in ("E", "B", "J"):
+        pairs = []
+        for c in "xyz":
+            for m in range(M + 1):
+                a = mx.solver.DataDev[f"{f}{c}_m{m}"][1:]
+                b = fp.solver.DataDev[f"{f}{c}_m{m}"][1:]
+                pairs.append((a.to(torch.complex128 if b.is_complex() else torch.float64), b))
+        out[f] = _rel(pairs)
+    out["p"] = _rel([(mx.eons.DataDev[k].double(), fp.eons.DataDev[k]) for k in ("px", "py", "pz")])
+    return out
+
+
+def add_div_scratch(solver):
+    import torch
+    a, D = solver.Args, solver.DataDev
+    for n in ["divJ"] + [f"Jd{c}" for c in "xyz"]:
+        for m in range(a["M"] + 1):
+            D[f"{n}_fb_m{m}"] = torch.zeros((a["Nr"] - 1, a["Nx"]), dtype=torch.complex128,
+                                            device=solver.comm.device)
+
+
+def charge_residual(calc, sim, rho_prev):
+    """|| (rho^(n+1) - rho^n)/dt + div J^(n+1/2) || / || div J || over the
+    spectral planes (smoothed rho and J of the last two steps), in FP64 with
+    the FP64 solver's operators (field_div, transformer_methods_cl.py:135-183)."""
+    import torch
+    M, D, Dc = sim.solver.Args["M"], sim.solver.DataDev, calc.DataDev
+    for c in "xyz":
+        for m in range(M + 1):
+            Dc[f"Jd{c}_fb_m{m}"].copy_(D[f"J{c}_fb_m{m}"].to(torch.complex128))
+    calc.field_div("Jd", "divJ")
+    dt = sim.solver.Args["dt"]
+    num = den = 0.0
+    for m in range(M + 1):
+        r = (D[f"rho_fb_m{m}"].to(torch.complex128) - rho_prev[m]) / dt + Dc[f"divJ_fb_m{m}"]
+        num += float((r.abs() ** 2).sum())
+        den += float((Dc[f"divJ_fb_m{m}"].abs() ** 2).sum())
+    return (num / den) ** 0.5
+
+
+def energies(sim):
+    """Field energy sum 2 pi r dr dx (F_0^2 + 2 sum_m |F_m|^2) over E and B
+    (rows 1..Nr-1) and electron kinetic energy sum w (gamma - 1), FP64."""
+    import torch
+    s, a = sim.solver, sim.solver.Args
+    M = a["M"]
+    r = torch.as_tensor(a["Rgrid"][1:], dtype=torch.float64, device=s.comm.device)[:, None]
+    dv = 2 * np.pi * a["dr"] * a["dx"]
+    wf = 0.0
+    for f in ("E", "B"):
+        for c in "xyz":
+            wf += float((r * s.DataDev[f"{f}{c}_m0"][1:].double() ** 2).sum()) * dv
+            for m in range(1, M + 1):
+                v = s.DataDev[f"{f}{c}_m{m}"][1:]
+                wf += 2 * float((r * (v.real.double() ** 2 + v.imag.double() ** 2)).sum()) * dv
+    d = sim.eons.DataDev
+    u2 = d["px"].double() ** 2 + d["py"].double() ** 2 + d["pz"].double() ** 2
+    wk = float((d["w"].double() * (torch.sqrt(1 + u2) - 1)).sum())
+    return {"field": wf, "kinetic": wk}
+
+
+def rho_copy(sim):
+    import torch
+    M = sim.solver.Args["M"]
+    return [sim.solver.DataDev[f"rho_fb_m{m}"].to(torch.complex128).clone() for m in range(M + 1)]
+
+
+# ---------------------------------------------------------------- kernels
+def kernel_table(sim, reps, w):
+    """HIP-event timings (on the stream the kernels run on) of the step's
+    phases, with their algorithmic HBM bytes (or DHT flops) per call."""
+    import torch
+    s, e = sim.solver, sim.eons
+    a = s.Args
+    Np, M, Nx, Nr = e.Args["Np"], a["M"], a["Nx"], a["Nr"]
+    Nk = Nr - 1
+    r1 = Nr * Nx * w * (1 + 2 * M)          # one real-space component, all modes
+    s1 = 2 * w * Nk * Nx * (M + 1)          # one spectral component, all modes
+    st = torch.cuda.current_stream()
+
+    def timeit(fn):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        fn()
+        for x, y in ev:
+            x.record(st)
+            fn()
+            y.record(st)
+        torch.cuda.synchronize()
+        return float(np.mean([x.elapsed_time(y) for x, y in ev])) * 1e-3
+
+    rows = {}
+
+    def add(name, t, nbytes=None, flops=None, peak=None):
+        row = {"avg_us": t * 1e6}
+        if nbytes is not None:
+            row["bytes"] = nbytes
+            row["GB/s"] = nbytes / t / 1e9
+            row["hbm_frac"] = row["GB/s"] / HBM_PEAK_GBS
+        if flops is not None:
+            row["flops"] = flops
+            row["TFLOP/s"] = flops / t / 1e12
+            if peak:
+                row["mfma_frac"] = row["TFLOP/s"] / peak
+        rows[name] = row
+
+    add("gather+push (a)", timeit(lambda: s._gather_and_push(e, ["E", "B"])), Np * 10 * w + 6 * r1)
+    add("push_x+cell keys (a,b)", timeit(lambda: e.push_coords(mode="half", grid=s)), Np * (10 * w + 4))
+    add("cell keys + radix sort (b)", timeit(lambda: e.index_sort(s)), Np * (3 * w + 12))
+    add("J deposit (b)", timeit(lambda: s.depose_currents(species=[e])), Np * (8 * w + 4) + 3 * r1)
+    add("rho deposit, electrons (b)", timeit(lambda: s.depose_scalar(e, "w", "rho", -1)), Np * (4 * w + 4) + r1)
+    dflops = lambda nf: nf * (1 + 2 * M) * 2.0 * Nk * Nk * Nx
+    mixed = s.comm.mixed
+    peak = MFMA_PEAK_TFS["f32"] if mixed else MFMA_PEAK_TFS["f64"]
+    add("forward transform rho,J + smoothing (c)",
+        timeit(lambda: s.fb_transform_smooth(scals=["rho"], vects=["J"])), 4 * r1 + 4 * s1, dflops(4), peak)
+    add("backward transform E,B (c)", timeit(lambda: s.fb_transform(vects=["E", "B"], dir=1)),
+        6 * s1 + 6 * r1, dflops(6), peak)
+    add("rot/w^2 (c)", timeit(lambda: s.restore_B_fb()), 6 * s1, 2 * (4 * M + 2) * 2.0 * Nk * Nk * Nx, peak)
+    add("PSATD + damping (d)", timeit(lambda: s.damp_edges(("E", "G"), advance_with=("J", "dN0", "dN1"))),
+        21 * s1 + 1.5 * s1)
+    return rows
+
+
+def gather_roofline(sim, reps, w):
+    """The dominant kernel (gather + Boris push), timed with HIP events on the
+    stream it runs on.  Algorithmic bytes per launch: each electron reads x,
+    y, z, px, py, pz and writes px, py, pz, g_inv (10 words)."""
+    import torch
+    s, e = sim.solver, sim.eons
+    st = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, b in ev:
+        a.record(st)
+        s._gather_and_push(e, ["E", "B"])
+        b.record(st)
+    torch.cuda.synchronize()
+    t = float(np.mean([a.elapsed_time(b) for a, b in ev])) * 1e-3
+    nbytes = e.Args["Np"] * 10 * w
+    achieved = nbytes / t / 1e9
+    return {"kernel": "k_gather_push", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "avg_launch_us": t * 1e6, "algorithmic_bytes": nbytes}
+
+
+# ---------------------------------------------------------------- differentiable form
+def diff_section():
+    import torch
+    import chimeracl_amd.adjoint as A
+    out = {}
+
+    def grads(sim, a0, R, kv, n, objective):
+        rg = sim.solver.Args["Rgrid"]
+        pa = torch.tensor(a0, dtype=torch.float64, requires_grad=True)
+        pR = torch.tensor(R, dtype=torch.float64, requires_grad=True)
+        pk = torch.tensor(kv, dtype=torch.float64, requires_grad=True)
+        L = objective(A.run(sim, pa, pR, pk, n), rg)
+        L.backward()
+        g = np.concatenate([[pa.grad.item(), pR.grad.item()], pk.grad.numpy()])
+        res = {"objective": float(L.detach()), "steps": n, "checkpoint_every": sim.ckpt,
+               "electrons": sim.Np, "forward_s": sim.timings["forward"],
+               "backward_s": sim.timings["backward"],
+               "backward_over_forward": sim.timings["backward"] / sim.timings["forward"]}
+        return g, res
+
+    def build(name, precision):
+        sim, a0, R, kv, n = A.from_config(name, precision)
+        sim.forward(a0, R, kv, 2)  # eager step + graph capture
+        return sim, a0, R, kv, n
+
+    def free(sim):
+        del sim
+        gc.collect()
+        torch.cuda.empty_cache()
+
+    gam = lambda outs, rg: A.objective_gamma2(*outs[:3], threshold=5.0, tau=0.25)
+    wake = lambda outs, rg: A.objective_wake_energy(list(outs[3:]), rg, ncols=256)
+
+    # C4, FP64: gradients against central finite differences (step 1e-4 relative)
+    sim, a0, R, kv, n = build("C4", "f64")
+    g64, res = grads(sim, a0, R, kv, n, gam)
+    p0 = np.concatenate([[a0, R], kv])
+    fd = np.zeros_like(p0)
+    t0 = time.time()
+    for j in range(len(p0)):
+        h = 1e-4 * abs(p0[j])
+        dp = np.zeros_like(p0)
+        dp[j] = h
+        with torch.no_grad():
+            fp = float(gam(sim.forward(*(p0 + dp)[:2], (p0 + dp)[2:], n), None))
+            fm = float(gam(sim.forward(*(p0 - dp)[:2], (p0 - dp)[2:], n), None))
+        fd[j] = (fp - fm) / (2 * h)
+    log(f"[bench] C4 finite differences ({2 * len(p0)} runs) in {time.time() - t0:.1f}s")
+    kn_fd, kn_g = fd[2:], g64[2:]
+    big = np.abs(kn_fd) > 1e-3 * np.abs(kn_fd).max()
+    res["fd_check"] = {
+        "step": "1e-4 relative, central",
+        "a0": {"adjoint": g64[0], "fd": fd[0], "rel_err": abs(g64[0] - fd[0]) / abs(fd[0])},
+        "w0": {"adjoint": g64[1], "fd": fd[1], "rel_err": abs(g64[1] - fd[1]) / abs(fd[1])},
+        "knots_max_err_over_max_grad": float(np.abs(kn_g - kn_fd).max() / np.abs(kn_fd).max()),
+        "knots_median_rel_err_significant": float(np.median(np.abs(kn_g - kn_fd)[big] / np.abs(kn_fd)[big])),
+        "knots_significant": int(big.sum()),
+        "knots": len(kn_fd)}
+    out["C4_f64"] = res
+    free(sim)
+    sim, a0, R, kv, n = build("C4", "mixed")
+    gmx, res = grads(sim, a0, R, kv, n, gam)
+    res["grad_err_vs_fp64_adjoint"] = _grad_errs(gmx, g64)
+    out["C4_mixed"] = res
+    free(sim)
+
+    # C5: the mixed adjoint against the FP64 adjoint, after 50 steps and
+    # after the full 400 (by then ~1e5 electrons have left the valid cells
+    # at the right edge, which makes the objective rough: see the docs)
+    ref = {}
+    sim, a0, R, kv, n = build("C5", "f64")
+    for ns in (50, n):
+        ref[ns] = grads(sim, a0, R, kv, ns, wake)
+    free(sim)
+    sim, a0, R, kv, n = build("C5", "mixed")
+    for ns in (50, n):
+        g, res = grads(sim, a0, R, kv, ns, wake)
+        res["grad_err_vs_fp64_adjoint"] = _grad_errs(g, ref[ns][0])
+        res["fp64"] = {k: ref[ns][1][k] for k in ("objective", "forward_s", "backward_s",
+                                                   "backward_over_forward")}
+        out[f"C5_mixed_{ns}steps"] = res
+    free(sim)
+    return out
+
+
+def _grad_errs(g, ref):
+    return {"a0": abs(g[0] - ref[0]) / abs(ref[0]), "w0": abs(g[1] - ref[1]) / abs(ref[1]),
+            "knots_rel_l2": float(np.linalg.norm(g[2:] - ref[2:]) / np.linalg.norm(ref[2:]))}
+
+
+# ---------------------------------------------------------------- main
 def main():
     args = parse()
     if args.gpus != 1:
@@ -123,48 +386,110 @@ def main():
     cfg = C.CONFIGS[args.config]
     M = cfg["M"]
     want_cpu = not args.no_cpu
-    g, solver, eons, ions, loop, orc = build_sim(args.config, args.precision, want_cpu)
-    Ne = eons.Args["Np"]
+    fp = Sim(args.config, "f64", want_cpu)
+    mx = None if args.no_mixed else Sim(args.config, "mixed", False)
+    sims = [s for s in (fp, mx) if s is not None]
+    Ne = fp.eons.Args["Np"]
+    add_div_scratch(fp.solver)
     torch.cuda.synchronize()
 
     # ---- CPU baseline (oracle) on a bounded sample: the first cpu_steps steps
-    cpu = None
-    errs = None
+    cpu = errs = None
     if want_cpu:
         from oracle import oracle as O
         nthreads = O.lib().orc_num_threads()
-        s, e, i, oloop = orc
+        s, e, i, oloop = fp.orc
         t0 = time.time()
         for _ in range(args.cpu_steps):
             oloop.step()
         tc = time.time() - t0
-        for _ in range(args.cpu_steps):
-            loop.step()  # eager first step, captured graph from the second
+        for sm in sims:
+            sm.step(args.cpu_steps)  # eager first step, captured graph from the second
         torch.cuda.synchronize()
-        errs = oracle_errors(solver, eons, orc, M)
+        errs = oracle_errors(fp, M)
         cpu = {"value": Ne * args.cpu_steps / tc, "unit": "particle-steps/s", "cores": nthreads,
                "kind": "port",
                "sample": f"{args.cpu_steps} full PIC steps of the same workload ({Ne} electrons, "
-                         f"{g['Nx']}x{g['Nr']}, M={M}) by the CPU FP64 oracle "
+                         f"{fp.g['Nx']}x{fp.g['Nr']}, M={M}) by the CPU FP64 oracle "
                          f"(C/OpenMP particle loops, numpy FFT/BLAS), {tc:.1f}s"}
         log(f"[bench] oracle {args.cpu_steps} steps in {tc:.1f}s on {nthreads} threads; errors {errs}")
-        del orc, s, e, i, oloop
+        fp.orc = None
+        del s, e, i, oloop
 
-    # ---- warmup + timed steps (graph replay)
-    for _ in range(args.warmup):
-        loop.step()
-    torch.cuda.synchronize()
-    t0 = time.time()
-    for _ in range(args.steps):
-        loop.step()
-    torch.cuda.synchronize()
-    dt = time.time() - t0
+    diag_at = [int(v) for v in args.diag_steps.split(",") if v]
+    diags = {}
+
+    def advance_to(n):
+        """Lockstep advance of both runs to step n, with the diagnostics of
+        the mixed run against the FP64 run at the requested steps."""
+        while fp.steps < n:
+            nxt = min([d for d in diag_at if d > fp.steps] + [n])
+            k = nxt - fp.steps
+            if nxt in diag_at and k >= 1:
+                for sm in sims:
+                    sm.step(k - 1)
+                prev = [rho_copy(sm) for sm in sims]
+                for sm in sims:
+                    sm.step(1)
+                torch.cuda.synchronize()
+                d = {"fp64": {"charge_residual": charge_residual(fp.solver, fp, prev[0]),
+                              "energy": energies(fp)}}
+                if mx is not None:
+                    d["mixed"] = {"charge_residual": charge_residual(fp.solver, mx, prev[1]),
+                                  "energy": energies(mx),
+                                  "rel_l2_vs_fp64": mixed_errors(mx, fp, M)}
+                diags[str(nxt)] = d
+                log(f"[bench] step {nxt}: {d}")
+            else:
+                for sm in sims:
+                    sm.step(k)
+
+    # ---- warmup, then the timed steps of each form (graph replay)
+    for sm in sims:
+        sm.step(args.warmup)
+    first_diag = min(diag_at) if diag_at else 0
+    advance_to(max(first_diag, fp.steps))
+    timing = {}
+    for name, sm in (("f64", fp), ("mixed", mx)):
+        if sm is None:
+            continue
+        torch.cuda.synchronize()
+        t0 = time.time()
+        for _ in range(args.steps):
+            sm.loop.step()
+        torch.cuda.synchronize()
+        dt = time.time() - t0
+        sm.steps += args.steps
+        timing[name] = dt
+        log(f"[bench] {name}: {args.steps} steps in {dt:.3f}s: {dt / args.steps * 1e3:.3f} ms/step, "
+            f"{Ne * args.steps / dt:.4g} particle-steps/s")
+    if mx is not None and mx.steps != fp.steps:
+        raise RuntimeError("lockstep broken")
+    if diag_at:
+        advance_to(max(max(diag_at), fp.steps))
+
+    dt = timing["f64"]
     ms = dt / args.steps * 1e3
     pps = Ne * args.steps / dt
-    log(f"[bench] {args.steps} steps in {dt:.3f}s: {ms:.3f} ms/step, {pps:.4g} particle-steps/s")
-
-    # ---- dominant kernel: gather + Boris push, timed with HIP events
-    roof = kernel_roofline(solver, eons, args.kernel_reps, 8 if args.precision == "f64" else 4)
+    roof = gather_roofline(fp, args.kernel_reps, 8)
+    mixed = None
+    if mx is not None:
+        dtm = timing["mixed"]
+        mixed = {"value": Ne * args.steps / dtm, "unit": "particle-steps/s",
+                 "ms_per_step": dtm / args.steps * 1e3, "pic_steps_per_s": args.steps / dtm,
+                 "speedup_vs_f64": dt / dtm, "dtype": "f32 particles/FFT, split-bf16x3 DHT",
+                 "gather_roofline": gather_roofline(mx, args.kernel_reps, 4)}
+    kernels = {"f64": kernel_table(fp, args.kernel_reps, 8)}
+    if mx is not None:
+        kernels["mixed"] = kernel_table(mx, args.kernel_reps, 4)
+    del fp, mx, sims
+    gc.collect()
+    torch.cuda.empty_cache()
+    diff = None
+    if not args.no_diff:
+        t0 = time.time()
+        diff = diff_section()
+        log(f"[bench] differentiable section in {time.time() - t0:.1f}s")
 
     out = {
         "metric": "particle-steps/s",
@@ -178,36 +503,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "f64" else "f32+bf16x3",
+        "dtype": "f64",
         "data": "synthetic (seeded uniform plasma, Gaussian laser)",
-        "config": {"workload": f"{args.config}: " + cfg["desc"], "Nz": g["Nx"], "Nr": g["Nr"],
-                   "modes": M + 1, "ppc": list(cfg["ppc"]), "electrons": Ne, "dt": g["dt"],
-                   "precision": args.precision},
+        "config": {"workload": f"{args.config}: " + cfg["desc"], "Nz": cfg["Nx"], "Nr": cfg["Nr"],
+                   "modes": M + 1, "ppc": list(cfg["ppc"]), "electrons": Ne, "dt": C.DZ},
         "fp64_oracle_error": errs,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "mixed": mixed,
+        "diagnostics": diags,
+        "kernels": kernels,
+        "differentiable": diff,
     }
     print(json.dumps(out), flush=True)
-
-
-def kernel_roofline(solver, eons, reps, w):
-    """Times the gather + push launch with HIP events on the stream it runs on.
-    Algorithmic bytes per launch: each electron reads x, y, z, px, py, pz and
-    writes px, py, pz, g_inv -- 10 words."""
-    import torch
-    st = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-    for a, b in ev:
-        a.record(st)
-        solver._gather_and_push(eons, ["E", "B"])
-        b.record(st)
-    torch.cuda.synchronize()
-    t = float(np.mean([a.elapsed_time(b) for a, b in ev])) * 1e-3
-    nbytes = eons.Args["Np"] * 10 * w
-    achieved = nbytes / t / 1e9
-    return {"kernel": "k_gather_push", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "avg_launch_us": t * 1e6, "algorithmic_bytes": nbytes}
 
 
 if __name__ == "__main__":

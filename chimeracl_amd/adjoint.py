@@ -487,3 +487,22 @@ def objective_wake_energy(Ex_planes, rgrid, ncols=256):
         v = Em[1:, -ncols:]
         tot = tot + 2.0 * (r * (v.real.double() ** 2 + v.imag.double() ** 2)).sum()
     return tot
+
+
+def from_config(name, precision="f64", dht="bf16x3", max_particles=None, ckpt=None, **over):
+    """DiffPIC for one of the workloads of chimeracl_amd.config (C4: Nz=1024,
+    Nr=128, 2x2x4 ppc, 32 knots, checkpoints every 10 steps; C5: Nz=4096,
+    Nr=256, 2x2x8 ppc, 64 knots, checkpoints every 20 steps).  Returns
+    (sim, a0, R, knot values, number of steps)."""
+    from . import config as C
+    cfg = dict(C.CONFIGS[name], **over)
+    g = C.grid_config(name, **over)
+    las = C.laser_config(name, **over)
+    comm = Communicator(precision=precision, dht=dht)
+    dx = (g["Xmax"] - g["Xmin"]) / (g["Nx"] - 1)
+    dr = g["Rmax"] / (g["Nr"] - 1.5)
+    eo, io = C.species_config(name, {"dx": dx, "dr": dr, "dt": g["dt"]}, **over)
+    plasma = C.load_plasma(g, cfg["ppc"], cfg["dens"], seed=0, max_particles=max_particles)
+    kx, kv = C.density_profile(name, g, **over)
+    sim = DiffPIC(g, las, (eo, io), plasma, kx, comm, ckpt=ckpt or cfg.get("ckpt", 10))
+    return sim, las["a0"], las["R"], np.asarray(kv, dtype=np.float64), cfg["steps"]

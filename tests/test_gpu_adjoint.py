@@ -496,3 +496,25 @@ def test_run_gradient_vs_finite_differences():
     print("adjoint", grads, "\nfd     ", fd, "\nerr", err)
     assert float(L) > 0
     assert err.max() < 1e-5, err
+
+
+def test_mixed_adjoint_matches_fp64_adjoint():
+    """The FP32 adjoint (split-BF16 DHT products) of the tiny run agrees with
+    the FP64 adjoint to the precision of the mixed forward."""
+    _require_gpu()
+    import chimeracl_amd.adjoint as A
+    out = {}
+    for prec in ("f64", "mixed"):
+        sim, las, knots = _tiny_diff(prec)
+        rg = sim.solver.Args["Rgrid"]
+        a0 = torch.tensor(las["a0"], dtype=torch.float64, requires_grad=True)
+        R = torch.tensor(las["R"], dtype=torch.float64, requires_grad=True)
+        k = torch.tensor(knots, dtype=torch.float64, requires_grad=True)
+        outs = A.run(sim, a0, R, k, 6)
+        L = A.objective_gamma2(*outs[:3], threshold=1.5, tau=0.25) + \
+            1e3 * A.objective_wake_energy(list(outs[3:]), rg, ncols=16)
+        L.backward()
+        out[prec] = np.concatenate([[a0.grad.item(), R.grad.item()], k.grad.numpy()])
+    g64, g32 = out["f64"], out["mixed"]
+    err = np.abs(g32 - g64) / np.abs(g64).max()
+    assert err.max() < 1e-3, (g64, g32)
