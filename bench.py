@@ -54,7 +54,21 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--diag-steps", default="100,500")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy three times (HBM counter calibration)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """A plain device-to-device copy of 1 GiB, three times: reads 1 GiB and
+    writes 1 GiB per launch, the reference for FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": a.numel() * 4, "launches": 3}), flush=True)
 
 
 def log(*a):
@@ -377,6 +391,9 @@ def _grad_errs(g, ref):
 # ---------------------------------------------------------------- main
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in "
                           "DESIGN.md but not implemented", "n_gpus": args.gpus}))

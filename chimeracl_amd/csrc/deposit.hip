@@ -30,20 +30,31 @@ struct PlanePtrs {
 };
 
 // ------------------------------------------------------------------ pass 1
-// One wave per DEP_CELLS consecutive cells.  Their particles occupy one
-// contiguous range of sorted slots, which the wave stages into LDS in
-// chunks of DEP_CAP slots with coalesced loads (sort_indx is close to the
-// identity for a cell-major plasma, so the gathers are nearly coalesced too).
-// Lane l < DEP_CELLS then sums the particles of cell c0 + l sequentially, in
-// sorted order, with the reference kernel's arithmetic.
-#define DEP_CELLS 16
-#define DEP_CAP 512
+// One wave per group of consecutive cells.  Their particles occupy one
+// contiguous range of sorted slots, which the wave stages into LDS in chunks
+// of CAP slots with coalesced loads (sort_indx is close to the identity for a
+// cell-major plasma, so the gathers are nearly coalesced too).  While staging,
+// each lane computes its particles' CIC weights C_ij (weight included), mode
+// factors ((y + i z)/r)^m and values exactly as the reference kernel does;
+// then one lane per (cell, component, corner) sums its accumulators over the
+// cell's particles sequentially, in sorted order.  Every accumulator sees the
+// same operands in the same order as in the reference kernel, so the sums are
+// bit-identical to it.
+//
+// KIND 0: rho (wp = w q); 1: J (wp = w g_inv q, values u); 2: field
+// adjoints of the gather (wp = 1, values in ux/uy/uz, modes weighted 2 e_m).
+template <int M, int KIND> struct DepCfg {
+  static constexpr bool VEC = KIND != 0;
+  static constexpr int NK = VEC ? 3 : 1;              // components
+  static constexpr int TPC = 4 * NK;                   // lanes per cell
+  static constexpr int CELLS = 64 / TPC;               // cells per wave
+  static constexpr int CAP = VEC ? 192 : 512;          // staged slots per chunk
+  static constexpr int NV = 4 + 2 * M + (VEC ? 3 : 0); // staged values per particle
+};
 // Lanes walk their own cell's slots, ~ppc apart: one pad element per 32
 // spreads them over the LDS banks.
 #define DEP_SW(t) ((t) + ((t) >> 5))
 
-// KIND 0: rho (wp = w q); 1: J (wp = w g_inv q, values u); 2: field
-// adjoints of the gather (wp = 1, values in ux/uy/uz, modes weighted 2 e_m).
 template <typename T, int M, int KIND>
 __global__ void __launch_bounds__(64)
     k_cell_sums(const uint32_t *__restrict__ sort_indx, const uint32_t *__restrict__ cell_offset,
@@ -51,60 +62,96 @@ __global__ void __launch_bounds__(64)
                 const T *__restrict__ ux, const T *__restrict__ uy, const T *__restrict__ uz,
                 const T *__restrict__ g_inv, const T *__restrict__ w, T charge, int Nx, int Nr,
                 T xmin, T dx_inv, T rmin, T dr_inv, T *__restrict__ part) {
-  constexpr bool VEC = KIND != 0;
-  constexpr int NK = VEC ? 3 : 1;
-  constexpr int NV = KIND == 1 ? 8 : (KIND == 2 ? 6 : 4);
-  __shared__ T st[NV][DEP_CAP + DEP_CAP / 32];
+  typedef DepCfg<M, KIND> Cfg;
+  constexpr bool VEC = Cfg::VEC;
+  constexpr int CAP = Cfg::CAP, NV = Cfg::NV, TPC = Cfg::TPC;
+  constexpr int MM = M > 0 ? M : 1;
+  constexpr int VE = 4, VU = 4 + 2 * M;  // first mode-factor / value slot
+  __shared__ T st[NV][CAP + CAP / 32];
   const int Nx_cell = Nx - 1, Nr_cell = Nr - 1;
   const int64_t ncell = (int64_t)Nx_cell * Nr_cell;
   const int lane = threadIdx.x;
-  const int64_t c0 = (int64_t)blockIdx.x * DEP_CELLS;
-  const int64_t c1 = (c0 + DEP_CELLS < ncell) ? c0 + DEP_CELLS : ncell;
+  const int64_t c0 = (int64_t)blockIdx.x * Cfg::CELLS;
+  const int64_t c1 = (c0 + Cfg::CELLS < ncell) ? c0 + Cfg::CELLS : ncell;
   const uint32_t S0 = cell_offset[c0], S1 = cell_offset[c1];
-  const int64_t c = c0 + lane;
-  bool act = lane < DEP_CELLS && c < ncell;
-  int ir = 0, ix = 0;
+  const int task = lane % TPC, k = task >> 2, corner = task & 3;
+  const int64_t c = c0 + lane / TPC;
+  bool act = lane < Cfg::CELLS * TPC && c < c1;
   uint32_t ip0 = 0, ip1 = 0;
   if (act) {
-    ir = (int)(c / Nx_cell);
-    ix = (int)(c - (int64_t)ir * Nx_cell);
+    const int ir = (int)(c / Nx_cell), ix = (int)(c - (int64_t)ir * Nx_cell);
     act = ix > 0 && ix < Nx_cell - 1 && ir < Nr_cell - 1;
     ip0 = cell_offset[c];
     ip1 = cell_offset[c + 1];
     act = act && ip0 != ip1;
   }
-  T a0[NK][2][2];
-  T ar[NK][M > 0 ? M : 1][2][2], ai[NK][M > 0 ? M : 1][2][2];
+  T a0 = 0, ar[MM], ai[MM];
 #pragma unroll
-  for (int k = 0; k < NK; k++)
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        a0[k][i][j] = 0;
-#pragma unroll
-        for (int m = 0; m < (M > 0 ? M : 1); m++) ar[k][m][i][j] = ai[k][m][i][j] = 0;
-      }
-  for (uint32_t cs = S0; cs < S1; cs += DEP_CAP) {
-    const uint32_t n = (S1 - cs < DEP_CAP) ? S1 - cs : DEP_CAP;
+  for (int m = 0; m < MM; m++) ar[m] = ai[m] = 0;
+  for (uint32_t cs = S0; cs < S1; cs += CAP) {
+    const uint32_t n = (S1 - cs < CAP) ? S1 - cs : CAP;
     for (uint32_t t = lane; t < n; t += 64) {
-      uint32_t p = sort_indx[cs + t];
+      const uint32_t p = sort_indx[cs + t];
       const uint32_t q = DEP_SW(t);
-      st[0][q] = x[p];
-      st[1][q] = y[p];
-      st[2][q] = z[p];
+      const T xp = x[p], yp = y[p], zp = z[p];
+      T wp, jp[3] = {0, 0, 0};
       if (KIND == 2) {
-        st[3][q] = ux[p];
-        st[4][q] = uy[p];
-        st[5][q] = uz[p];
+        jp[0] = ux[p];
+        jp[1] = uy[p];
+        jp[2] = uz[p];
+        wp = (T)1;
+      } else if (KIND == 1) {
+        jp[0] = ux[p];
+        jp[1] = uy[p];
+        jp[2] = uz[p];
+        wp = w[p] * g_inv[p] * charge;
       } else {
-        st[3][q] = w[p];
+        wp = w[p] * charge;
       }
-      if (KIND == 1) {
-        st[4][q] = ux[p];
-        st[5][q] = uy[p];
-        st[6][q] = uz[p];
-        st[7][q] = g_inv[p];
+      const T rp = dsqrt<T>(yp * yp + zp * zp);
+      T rp_inv;
+      if (VEC) {
+        rp_inv = 0;
+        if (rp > 0) rp_inv = (T)1. / rp;
+      } else {
+        rp_inv = (T)1. / rp;
+      }
+      // ((y + i z)/r)^m, m = 1..M
+      T er[MM + 1], ei[MM + 1];
+      er[1] = yp * rp_inv;
+      ei[1] = zp * rp_inv;
+#pragma unroll
+      for (int m = 2; m <= M; m++) {
+        er[m] = er[m - 1] * er[1] - ei[m - 1] * ei[1];
+        ei[m] = er[m - 1] * ei[1] + ei[m - 1] * er[1];
+      }
+      if (KIND == 2) {
+#pragma unroll
+        for (int m = 1; m <= M; m++) {
+          er[m] = (T)2 * er[m];
+          ei[m] = (T)2 * ei[m];
+        }
+      }
+      // the particle's cell is the one its key names
+      const T fx = (xp - xmin) * dx_inv, fr = (rp - rmin) * dr_inv;
+      const T sX1 = fx - dfloor<T>(fx), sR1 = fr - dfloor<T>(fr);
+      T sX0 = (T)1.0 - sX1;
+      const T sR0 = (T)1.0 - sR1;
+      T sXw1 = sX1;
+      sX0 *= wp;
+      sXw1 *= wp;
+      st[0][q] = sR0 * sX0;
+      st[1][q] = sR0 * sXw1;
+      st[2][q] = sR1 * sX0;
+      st[3][q] = sR1 * sXw1;
+#pragma unroll
+      for (int m = 1; m <= M; m++) {
+        st[VE + 2 * (m - 1)][q] = er[m];
+        st[VE + 2 * (m - 1) + 1][q] = ei[m];
+      }
+      if (VEC) {
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) st[VU + kk][q] = jp[kk];
       }
     }
     __syncthreads();
@@ -112,80 +159,24 @@ __global__ void __launch_bounds__(64)
       uint32_t lo = ip0 > cs ? ip0 : cs, hi = ip1 < cs + n ? ip1 : cs + n;
       for (uint32_t ip = lo; ip < hi; ip++) {
         const uint32_t t = DEP_SW(ip - cs);
-        T xp = st[0][t], yp = st[1][t], zp = st[2][t];
-        T jp[3];
-        T wp;
-        if (KIND == 2) {
-          jp[0] = st[3][t];
-          jp[1] = st[4][t];
-          jp[2] = st[5][t];
-          wp = (T)1;
-        } else if (KIND == 1) {
-          jp[0] = st[4][t];
-          jp[1] = st[5][t];
-          jp[2] = st[6][t];
-          wp = st[3][t] * st[7][t] * charge;
-        } else {
-          wp = st[3][t] * charge;
-        }
-        T rp = dsqrt<T>(yp * yp + zp * zp);
-        T rp_inv;
+        const T C = st[corner][t];
         if (VEC) {
-          rp_inv = 0;
-          if (rp > 0) rp_inv = (T)1. / rp;
-        } else {
-          rp_inv = (T)1. / rp;
-        }
-        // ((y + i z)/r)^m, m = 1..M
-        T er[M > 0 ? M + 1 : 2], ei[M > 0 ? M + 1 : 2];
-        er[1] = yp * rp_inv;
-        ei[1] = zp * rp_inv;
-#pragma unroll
-        for (int m = 2; m <= M; m++) {
-          er[m] = er[m - 1] * er[1] - ei[m - 1] * ei[1];
-          ei[m] = er[m - 1] * ei[1] + ei[m - 1] * er[1];
-        }
-        if (KIND == 2) {
+          const T u = st[VU + k][t];
+          const T jproj = C * u;
+          a0 += C * u;
 #pragma unroll
           for (int m = 1; m <= M; m++) {
-            er[m] = (T)2 * er[m];
-            ei[m] = (T)2 * ei[m];
+            ar[m - 1] += jproj * st[VE + 2 * (m - 1)][t];
+            ai[m - 1] += jproj * st[VE + 2 * (m - 1) + 1][t];
+          }
+        } else {
+          a0 += C;
+#pragma unroll
+          for (int m = 1; m <= M; m++) {
+            ar[m - 1] += C * st[VE + 2 * (m - 1)][t];
+            ai[m - 1] += C * st[VE + 2 * (m - 1) + 1][t];
           }
         }
-        T sX1 = (xp - xmin) * dx_inv - (T)ix;
-        T sX0 = (T)1.0 - sX1;
-        T sR1 = (rp - rmin) * dr_inv - (T)ir;
-        T sR0 = (T)1.0 - sR1;
-        sX0 *= wp;
-        sX1 *= wp;
-        T C[2][2];
-        C[0][0] = sR0 * sX0;
-        C[0][1] = sR0 * sX1;
-        C[1][0] = sR1 * sX0;
-        C[1][1] = sR1 * sX1;
-#pragma unroll
-        for (int k = 0; k < NK; k++)
-#pragma unroll
-          for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-              if (VEC) {
-                T jproj = C[i][j] * jp[k];
-                a0[k][i][j] += C[i][j] * jp[k];
-#pragma unroll
-                for (int m = 1; m <= M; m++) {
-                  ar[k][m - 1][i][j] += jproj * er[m];
-                  ai[k][m - 1][i][j] += jproj * ei[m];
-                }
-              } else {
-                a0[k][i][j] += C[i][j];
-#pragma unroll
-                for (int m = 1; m <= M; m++) {
-                  ar[k][m - 1][i][j] += C[i][j] * er[m];
-                  ai[k][m - 1][i][j] += C[i][j] * ei[m];
-                }
-              }
-            }
       }
     }
     __syncthreads();
@@ -193,20 +184,12 @@ __global__ void __launch_bounds__(64)
   if (!act) return;
   // staging layout: part[((k*(1+2M) + q)*4 + corner) * ncell + c]
   constexpr int NQ = 1 + 2 * M;
+  part[((int64_t)(k * NQ) * 4 + corner) * ncell + c] = a0;
 #pragma unroll
-  for (int k = 0; k < NK; k++)
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        int corner = 2 * i + j;
-        part[((int64_t)(k * NQ) * 4 + corner) * ncell + c] = a0[k][i][j];
-#pragma unroll
-        for (int m = 1; m <= M; m++) {
-          part[((int64_t)(k * NQ + 2 * m - 1) * 4 + corner) * ncell + c] = ar[k][m - 1][i][j];
-          part[((int64_t)(k * NQ + 2 * m) * 4 + corner) * ncell + c] = ai[k][m - 1][i][j];
-        }
-      }
+  for (int m = 1; m <= M; m++) {
+    part[((int64_t)(k * NQ + 2 * m - 1) * 4 + corner) * ncell + c] = ar[m - 1];
+    part[((int64_t)(k * NQ + 2 * m) * 4 + corner) * ncell + c] = ai[m - 1];
+  }
 }
 
 // ------------------------------------------------------------------ pass 2
@@ -273,7 +256,7 @@ static void launch_deposit(const uint32_t *sort_indx, const uint32_t *cell_offse
                            const PlanePtrs &pp, T *part, bool pass1, hipStream_t st) {
   int64_t ncell = (int64_t)(g->Nx - 1) * (g->Nr - 1);
   if (pass1)
-    k_cell_sums<T, M, KIND><<<chm_blocks(ncell, DEP_CELLS), 64, 0, st>>>(
+    k_cell_sums<T, M, KIND><<<chm_blocks(ncell, DepCfg<M, KIND>::CELLS), 64, 0, st>>>(
       sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr, (T)g->xmin,
       (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, part);
   int64_t nnode = (int64_t)(g->Nr - 1) * g->Nx;
@@ -590,10 +573,170 @@ __global__ void __launch_bounds__(256)
   g_inv[p] = g_p_inv;
 }
 
+// The same gather + push visiting sorted slots ip < Np_stay with
+// p = sort_indx[ip] (the reference kernel's own iteration, grid_deposit_m1.cl:
+// 330-350, skipping p >= Np_stay).  A block of GS_NT slots covers a short run
+// of consecutive cells; when it spans at most MAXC cells, the E and B node
+// values of those cells (4 nodes x NV values each) are staged into LDS once
+// and every lane reads its 72 (M = 1) field values from there instead of from
+// 18 separate planes.  Identical operands and operation order: the results
+// equal k_gather_push bit for bit.
+#define GS_NT 256
+template <typename T, int M> struct GatherCfg {
+  static constexpr int NQ = 1 + 2 * M;
+  static constexpr int NV = 6 * NQ;  // E and B, 3 components, all modes
+  static constexpr int MC = 20480 / (4 * NV * (int)sizeof(T));
+  static constexpr int MAXC = MC > 4 ? MC : 4;
+};
+
+template <typename T, int M>
+__device__ __forceinline__ T field_val(const FieldPtrs &fp, int v, int64_t node) {
+  typedef GatherCfg<T, M> Cfg;
+  const int f = v / (3 * Cfg::NQ), r = v - f * 3 * Cfg::NQ;
+  const int k = r / Cfg::NQ, q = r - k * Cfg::NQ;
+  if (q == 0) return ((const T *)(f ? fp.b0[k] : fp.e0[k]))[node];
+  const int m = (q + 1) >> 1;
+  return ((const T *)(f ? fp.bc[k][m - 1] : fp.ec[k][m - 1]))[2 * node + ((q - 1) & 1)];
+}
+
+// E_p, B_p of one particle from its 4 nodes (value v of node (i, j) from
+// ld(v, i, j), values ordered as in GatherCfg), in k_gather_push's order.
+template <typename T, int M, typename LD>
+__device__ __forceinline__ void gather_sum(LD ld, const T (&C)[2][2], const T *gr, const T *gi,
+                                           T *e_p, T *b_p) {
+  constexpr int NQ = 1 + 2 * M;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        e_p[k] += C[i][j] * ld(k * NQ, i, j);
+        b_p[k] += C[i][j] * ld(3 * NQ + k * NQ, i, j);
+#pragma unroll
+        for (int m = 1; m <= M; m++) {
+          T er = 2 * ld(k * NQ + 2 * m - 1, i, j), ei = 2 * ld(k * NQ + 2 * m, i, j);
+          T br = 2 * ld(3 * NQ + k * NQ + 2 * m - 1, i, j), bi = 2 * ld(3 * NQ + k * NQ + 2 * m, i, j);
+          e_p[k] += C[i][j] * er * gr[m];
+          e_p[k] -= C[i][j] * ei * gi[m];
+          b_p[k] += C[i][j] * br * gr[m];
+          b_p[k] -= C[i][j] * bi * gi[m];
+        }
+      }
+}
+
+template <typename T, int M>
+__global__ void __launch_bounds__(GS_NT)
+    k_gather_sorted(const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+                    T *__restrict__ px, T *__restrict__ py, T *__restrict__ pz,
+                    T *__restrict__ g_inv, const uint32_t *__restrict__ sort_indx,
+                    const uint32_t *__restrict__ np_stay_p, T factor_push, int Nx, int Nr, T xmin,
+                    T dx_inv, T rmin, T dr_inv, FieldPtrs fp) {
+  typedef GatherCfg<T, M> Cfg;
+  constexpr int NV = Cfg::NV, NQ = Cfg::NQ, MAXC = Cfg::MAXC;
+  __shared__ T fs[MAXC * 4 * NV];
+  __shared__ int64_t cab[2];
+  const int64_t np_stay = (int64_t)(*np_stay_p);
+  const int64_t s0 = (int64_t)blockIdx.x * GS_NT;
+  if (s0 >= np_stay) return;  // uniform over the block
+  const int Nx_cell = Nx - 1;
+  const int64_t slot = s0 + threadIdx.x;
+  const bool live = slot < np_stay;
+  const int64_t nlive = (np_stay - s0 < GS_NT) ? np_stay - s0 : GS_NT;
+  uint32_t p = 0;
+  T xp = 0, yp = 0, zp = 0, rp = 0;
+  int ix = 0, ir = 0;
+  if (live) {
+    p = sort_indx[slot];
+    xp = x[p];
+    yp = y[p];
+    zp = z[p];
+    rp = dsqrt<T>(yp * yp + zp * zp);
+    ix = (int)dfloor<T>((xp - xmin) * dx_inv);
+    ir = (int)dfloor<T>((rp - rmin) * dr_inv);
+  }
+  // slots below Np_stay hold particles of valid cells, in cell order
+  const int64_t cell = ix + (int64_t)ir * Nx_cell;
+  if (threadIdx.x == 0) cab[0] = cell;
+  if (threadIdx.x == nlive - 1) cab[1] = cell;
+  __syncthreads();
+  const int64_t cA = cab[0], ncl = cab[1] - cA + 1;
+  const bool staged = ncl <= MAXC;
+  if (staged) {
+    const int tot = (int)ncl * 4 * NV;
+    for (int e = threadIdx.x; e < tot; e += GS_NT) {
+      const int v = e / ((int)ncl * 4), rem = e - v * (int)ncl * 4;
+      const int cl = rem >> 2, corner = rem & 3;
+      const int64_t cc = cA + cl;
+      const int irc = (int)(cc / Nx_cell), ixc = (int)(cc - (int64_t)irc * Nx_cell);
+      const int64_t node = ixc + (corner & 1) + (int64_t)(irc + (corner >> 1)) * Nx;
+      fs[(cl * 4 + corner) * NV + v] = field_val<T, M>(fp, v, node);
+    }
+  }
+  __syncthreads();
+  if (!live || !((int64_t)p < np_stay)) return;
+  const int64_t igrid = ix + (int64_t)ir * Nx;
+  const int lc = (int)(cell - cA);
+  T u[3] = {px[p], py[p], pz[p]};
+  T dt_2 = (T)0.5 * factor_push;
+  T rp_inv = (T)1. / rp;
+  T sX1 = (xp - xmin) * dx_inv - (T)ix;
+  T sX0 = (T)1.0 - sX1;
+  T sR1 = (rp - rmin) * dr_inv - (T)ir;
+  T sR0 = (T)1.0 - sR1;
+  T gr[M > 0 ? M + 1 : 2], gi[M > 0 ? M + 1 : 2];
+  gr[1] = yp * rp_inv;
+  gi[1] = -zp * rp_inv;
+#pragma unroll
+  for (int m = 2; m <= M; m++) {
+    gr[m] = gr[m - 1] * gr[1] - gi[m - 1] * gi[1];
+    gi[m] = gr[m - 1] * gi[1] + gi[m - 1] * gr[1];
+  }
+  T C[2][2];
+  C[0][0] = sR0 * sX0;
+  C[0][1] = sR0 * sX1;
+  C[1][0] = sR1 * sX0;
+  C[1][1] = sR1 * sX1;
+  T e_p[3] = {0, 0, 0}, b_p[3] = {0, 0, 0};
+  if (staged) {
+    const T *f0 = fs + lc * 4 * NV;
+    gather_sum<T, M>([&](int v, int i, int j) { return f0[(2 * i + j) * NV + v]; }, C, gr, gi,
+                     e_p, b_p);
+  } else {
+    gather_sum<T, M>([&](int v, int i, int j) {
+      return field_val<T, M>(fp, v, igrid + j + (int64_t)Nx * i);
+    }, C, gr, gi, e_p, b_p);
+  }
+  // Boris rotation, grid_deposit_m1.cl:472-507
+  T um[3], up[3], u0[3], t[3], s[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) um[k] = u[k] + dt_2 * e_p[k];
+  T g_p_inv = (T)1. / dsqrt<T>((T)1. + um[0] * um[0] + um[1] * um[1] + um[2] * um[2]);
+#pragma unroll
+  for (int k = 0; k < 3; k++) t[k] = dt_2 * b_p[k] * g_p_inv;
+  T t2 = (T)2. / ((T)1. + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+#pragma unroll
+  for (int k = 0; k < 3; k++) s[k] = t[k] * t2;
+  u0[0] = um[0] + um[1] * t[2] - um[2] * t[1];
+  u0[1] = um[1] - um[0] * t[2] + um[2] * t[0];
+  u0[2] = um[2] + um[0] * t[1] - um[1] * t[0];
+  up[0] = um[0] + u0[1] * s[2] - u0[2] * s[1];
+  up[1] = um[1] - u0[0] * s[2] + u0[2] * s[0];
+  up[2] = um[2] + u0[0] * s[1] - u0[1] * s[0];
+#pragma unroll
+  for (int k = 0; k < 3; k++) u[k] = up[k] + dt_2 * e_p[k];
+  g_p_inv = (T)1. / dsqrt<T>((T)1. + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  px[p] = u[0];
+  py[p] = u[1];
+  pz[p] = u[2];
+  g_inv[p] = g_p_inv;
+}
+
 template <typename T>
 static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, T *g_inv,
-                       int64_t Np, const uint32_t *np_stay, T factor_push, const chm_grid_t *g,
-                       const chm_planes_t *E, const chm_planes_t *B, void *stream) {
+                       int64_t Np, const uint32_t *sort_indx, const uint32_t *np_stay,
+                       T factor_push, const chm_grid_t *g, const chm_planes_t *E,
+                       const chm_planes_t *B, void *stream) {
   if (!g || !E || !B || g->M < 0 || g->M > CHM_MAXM || Np < 0) return CHM_EINVAL;
   if (Np == 0) return CHM_OK;
   FieldPtrs fp = {};
@@ -610,9 +753,14 @@ static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, 
   switch (g->M) {
 #define CASE(MM)                                                                                   \
   case MM:                                                                                         \
-    k_gather_push<T, MM><<<nb, 256, 0, st>>>(x, y, z, px, py, pz, g_inv, Np, np_stay, factor_push, \
-                                             g->Nx, g->Nr, (T)g->xmin, (T)g->dx_inv, (T)g->rmin,   \
-                                             (T)g->dr_inv, fp);                                    \
+    if (sort_indx)                                                                                 \
+      k_gather_sorted<T, MM><<<chm_blocks(Np, GS_NT), GS_NT, 0, st>>>(                             \
+          x, y, z, px, py, pz, g_inv, sort_indx, np_stay, factor_push, g->Nx, g->Nr, (T)g->xmin,   \
+          (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp);                                             \
+    else                                                                                           \
+      k_gather_push<T, MM><<<nb, 256, 0, st>>>(x, y, z, px, py, pz, g_inv, Np, np_stay,            \
+                                               factor_push, g->Nx, g->Nr, (T)g->xmin,              \
+                                               (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp);        \
     break;
     CASE(0) CASE(1) CASE(2) CASE(3) CASE(4)
 #undef CASE
@@ -623,14 +771,17 @@ static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, 
 
 extern "C" int chm_gather_push_f64(const double *x, const double *y, const double *z, double *px,
                                    double *py, double *pz, double *g_inv, int64_t Np,
-                                   const uint32_t *np_stay, double factor_push,
-                                   const chm_grid_t *g, const chm_planes_t *E,
+                                   const uint32_t *sort_indx, const uint32_t *np_stay,
+                                   double factor_push, const chm_grid_t *g, const chm_planes_t *E,
                                    const chm_planes_t *B, void *stream) {
-  return gather_push<double>(x, y, z, px, py, pz, g_inv, Np, np_stay, factor_push, g, E, B, stream);
+  return gather_push<double>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, factor_push, g, E,
+                             B, stream);
 }
 extern "C" int chm_gather_push_f32(const float *x, const float *y, const float *z, float *px,
                                    float *py, float *pz, float *g_inv, int64_t Np,
-                                   const uint32_t *np_stay, float factor_push, const chm_grid_t *g,
-                                   const chm_planes_t *E, const chm_planes_t *B, void *stream) {
-  return gather_push<float>(x, y, z, px, py, pz, g_inv, Np, np_stay, factor_push, g, E, B, stream);
+                                   const uint32_t *sort_indx, const uint32_t *np_stay,
+                                   float factor_push, const chm_grid_t *g, const chm_planes_t *E,
+                                   const chm_planes_t *B, void *stream) {
+  return gather_push<float>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, factor_push, g, E,
+                            B, stream);
 }

@@ -8,9 +8,11 @@
 //      run of equal keys inside a wave, no return value used)
 //   2. exclusive scan of the histogram -> cell_offset, Np_stay
 //   3. stable LSD radix sort of the keys with the particle index as value,
-//      8-bit digits: per 4096-key tile an LDS digit histogram, one scan over
-//      [digit][tile], then a scatter that ranks keys inside the tile in index
-//      order (ballot-matched digits per wave, tagged per-wave counts in LDS).
+//      in one to three passes of 8..11-bit digits (two passes up to 22-bit
+//      keys, i.e. 4M cells): per 4096-key tile an LDS digit histogram, one
+//      scan over [digit][tile], then a scatter that ranks keys inside the
+//      tile in index order (ballot-matched digits per wave, tagged per-wave
+//      counts in LDS).
 // The result is exactly numpy.argsort(key, kind='stable'); no float or
 // returning global atomics, no order fix-up.
 #include "chm_common.h"
@@ -225,11 +227,16 @@ __global__ void k_offsets(const uint32_t *__restrict__ pre, uint32_t nbins, uint
 }
 
 // ------------------------------------------------------------------ radix sort
+// Digits of nbits <= 2^RB buckets; counts[d * nblk + tile].
+template <int RB>
 __global__ void __launch_bounds__(RS_BS) k_rs_upsweep(const uint32_t *__restrict__ keys, int64_t n,
-                                                      int shift, uint32_t mask,
+                                                      int shift, int nbits,
                                                       uint32_t *__restrict__ counts, int64_t nblk) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  constexpr int NB = 1 << RB;
+  __shared__ uint32_t h[NB];
+  const int nd = 1 << nbits;
+  const uint32_t mask = (uint32_t)nd - 1u;
+  for (int d = threadIdx.x; d < nd; d += RS_BS) h[d] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
 #pragma unroll 4
@@ -238,7 +245,7 @@ __global__ void __launch_bounds__(RS_BS) k_rs_upsweep(const uint32_t *__restrict
     if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
   }
   __syncthreads();
-  counts[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+  for (int d = threadIdx.x; d < nd; d += RS_BS) counts[(int64_t)d * nblk + blockIdx.x] = h[d];
 }
 
 // Stable scatter of one tile.  Items are visited in index order: round r,
@@ -246,18 +253,23 @@ __global__ void __launch_bounds__(RS_BS) k_rs_upsweep(const uint32_t *__restrict
 // with equal digits are matched by ballots; per-wave digit counts go to LDS
 // tagged with the round number (no clearing needed); run[d] is the tile's
 // running output position of digit d.
+template <int RB>
 __global__ void __launch_bounds__(RS_BS) k_rs_downsweep(const uint32_t *__restrict__ kin,
                                                         const uint32_t *__restrict__ vin, int64_t n,
                                                         int shift, int nbits,
                                                         const uint32_t *__restrict__ offs, int64_t nblk,
                                                         uint32_t *__restrict__ kout,
                                                         uint32_t *__restrict__ vout) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wc[RS_BS / CHM_WAVE][256];
+  constexpr int NB = 1 << RB;
+  __shared__ uint32_t run[NB];
+  __shared__ uint32_t wc[RS_BS / CHM_WAVE][NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint32_t mask = (1u << nbits) - 1u;
-  run[tid] = offs[(int64_t)tid * nblk + blockIdx.x];
-  for (int w = 0; w < RS_BS / CHM_WAVE; w++) wc[w][tid] = 0;
+  const int nd = 1 << nbits;
+  const uint32_t mask = (uint32_t)nd - 1u;
+  for (int d = tid; d < nd; d += RS_BS) {
+    run[d] = offs[(int64_t)d * nblk + blockIdx.x];
+    for (int w = 0; w < RS_BS / CHM_WAVE; w++) wc[w][d] = 0;
+  }
   __syncthreads();
   const unsigned long long lt = (1ull << lane) - 1ull;
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
@@ -296,6 +308,35 @@ __global__ void __launch_bounds__(RS_BS) k_rs_downsweep(const uint32_t *__restri
   }
 }
 
+// Pass plan for keys of `bits` bits: one or two 8-bit passes up to 16 bits,
+// then two (up to 22 bits) or three passes of at most 11 bits.
+struct RsPlan {
+  int npass, rb, nb[3];
+};
+static RsPlan rs_plan(int bits) {
+  RsPlan p = {};
+  if (bits < 1) bits = 1;
+  if (bits <= 16) {
+    p.rb = 8;
+    p.npass = (bits + 7) / 8;
+  } else {
+    p.rb = 11;
+    p.npass = bits <= 22 ? 2 : 3;
+  }
+  int left = bits;
+  for (int i = 0; i < p.npass; i++) {
+    int w = (left + (p.npass - i) - 1) / (p.npass - i);
+    p.nb[i] = w;
+    left -= w;
+  }
+  return p;
+}
+static int key_bits(int64_t nbins) {
+  int bits = 0;
+  while (((int64_t)1 << bits) < nbins) bits++;  // keys are < nbins
+  return bits < 1 ? 1 : bits;
+}
+
 // ------------------------------------------------------------------ driver
 struct SortWs {
   uint32_t *hist, *pre, *bsum, *counts, *offs, *bsum2, *kA, *kB, *vA, *vB;
@@ -305,7 +346,7 @@ struct SortWs {
 static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
   int64_t nblk = (Np + RS_TILE - 1) / RS_TILE;
   if (nblk < 1) nblk = 1;
-  int64_t ncounts = 256 * nblk;
+  int64_t ncounts = ((int64_t)1 << rs_plan(key_bits(nbins)).rb) * nblk;
   int64_t nscan = (nbins + SCAN_TILE - 1) / SCAN_TILE;
   int64_t nscan2 = (ncounts + SCAN_TILE - 1) / SCAN_TILE;
   size_t off = 0;
@@ -383,21 +424,30 @@ static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_
     CHM_CHECK_LAUNCH();
     return CHM_OK;
   }
-  int bits = 0;
-  while (((int64_t)1 << bits) < nbins) bits++;  // keys are < nbins
-  if (bits < 1) bits = 1;
-  const int npass = (bits + 7) / 8;
+  const RsPlan plan = rs_plan(key_bits(nbins));
   const uint32_t *kin = indx_in_cell, *vin = nullptr;
-  for (int p = 0; p < npass; p++) {
-    const int shift = 8 * p, nb = (bits - shift < 8) ? bits - shift : 8;
-    const bool last = p == npass - 1;
+  int shift = 0;
+  for (int p = 0; p < plan.npass; p++) {
+    const int nb = plan.nb[p];
+    const bool last = p == plan.npass - 1;
     uint32_t *kout = last ? nullptr : ((p & 1) ? w.kB : w.kA);
     uint32_t *vout = last ? sort_indx : ((p & 1) ? w.vB : w.vA);
-    k_rs_upsweep<<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, (1u << nb) - 1u, w.counts, w.nblk);
-    scan_u32(w.counts, w.ncounts, w.offs, w.bsum2, w.nscan2, st);
-    k_rs_downsweep<<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs, w.nblk, kout, vout);
+    const int64_t nc = ((int64_t)1 << nb) * w.nblk;
+    const int64_t nsc = (nc + SCAN_TILE - 1) / SCAN_TILE;
+    if (plan.rb == 8)
+      k_rs_upsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
+    else
+      k_rs_upsweep<11><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
+    scan_u32(w.counts, nc, w.offs, w.bsum2, nsc, st);
+    if (plan.rb == 8)
+      k_rs_downsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs, w.nblk,
+                                                            kout, vout);
+    else
+      k_rs_downsweep<11><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs,
+                                                             w.nblk, kout, vout);
     kin = kout;
     vin = vout;
+    shift += nb;
   }
   CHM_CHECK_LAUNCH();
   return CHM_OK;

@@ -112,7 +112,8 @@ class GridMethodsHIP(GenericMethodsHIP):
         self.call("chm_warp_axis", self._planes_arr(names), len(names), a["Nx"], a["Nr"], a["M"])
 
     def _gather_and_push(self, parts, flds):
-        """grid_methods_cl.py:168-192."""
+        """grid_methods_cl.py:168-192 (visits the sorted slots, as the
+        reference kernel does; the species must be sorted for this grid)."""
         if parts.Args["Np"] == 0:
             return
         d = parts.DataDev
@@ -121,4 +122,5 @@ class GridMethodsHIP(GenericMethodsHIP):
         B = self._planes_arr([flds[1] + c for c in comps])
         self.call("chm_gather_push", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]),
                   ptr(d["py"]), ptr(d["pz"]), ptr(d["g_inv"]), parts.Args["Np"],
-                  ptr(d["Np_stay"]), parts.Args["FactorPush"], self._grid(), E, B)
+                  ptr(d["sort_indx"]), ptr(d["Np_stay"]), parts.Args["FactorPush"],
+                  self._grid(), E, B)

@@ -262,16 +262,22 @@ int chm_warp_axis_f32(const chm_planes_t *flds, int32_t nflds, int32_t Nx,
                       int32_t Nr, int32_t M, void *stream);
 /* Field gather and relativistic Boris push of every particle p with
  * p < *np_stay inside a valid cell (the set the reference's per-slot launch
- * visits).  factor_push = Args['FactorPush'] = 2*pi*dt*q/m. */
+ * visits).  factor_push = Args['FactorPush'] = 2*pi*dt*q/m.  sort_indx: the
+ * current sorted order (chm_index_sort_* / chm_sort_from_keys of these
+ * positions); the kernel then visits slots ip < *np_stay with
+ * p = sort_indx[ip] as the reference does and stages the fields of each
+ * slot block's cells in LDS.  NULL: one lane per particle index, fields
+ * read directly.  Both give identical results. */
 int chm_gather_push_f64(const double *x, const double *y, const double *z,
                         double *px, double *py, double *pz, double *g_inv,
-                        int64_t Np, const uint32_t *np_stay,
-                        double factor_push, const chm_grid_t *g,
-                        const chm_planes_t *E, const chm_planes_t *B,
-                        void *stream);
+                        int64_t Np, const uint32_t *sort_indx,
+                        const uint32_t *np_stay, double factor_push,
+                        const chm_grid_t *g, const chm_planes_t *E,
+                        const chm_planes_t *B, void *stream);
 int chm_gather_push_f32(const float *x, const float *y, const float *z,
                         float *px, float *py, float *pz, float *g_inv,
-                        int64_t Np, const uint32_t *np_stay, float factor_push,
+                        int64_t Np, const uint32_t *sort_indx,
+                        const uint32_t *np_stay, float factor_push,
                         const chm_grid_t *g, const chm_planes_t *E,
                         const chm_planes_t *B, void *stream);
 
