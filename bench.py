@@ -88,7 +88,7 @@ class Sim:
         CA.add_gausian_pulse(self.solver, dict(laser))
         eo, io = C.species_config(cfg_name, self.solver.Args)
         t = time.time()
-        arr = C.load_plasma(g, c["ppc"], c["dens"], seed=0)
+        arr = C.load_plasma(g, c["ppc"], c["dens"], seed=0, order="cell")
         log(f"[bench] {precision}: loaded {arr['x'].size} particles in {time.time() - t:.1f}s")
         self.eons, self.ions = CA.Particles(dict(eo), self.comm), CA.Particles(dict(io), self.comm)
         self.eons.add_particles(source=C.ParticleSource(arr))

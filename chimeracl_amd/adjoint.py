@@ -502,7 +502,8 @@ def from_config(name, precision="f64", dht="bf16x3", max_particles=None, ckpt=No
     dx = (g["Xmax"] - g["Xmin"]) / (g["Nx"] - 1)
     dr = g["Rmax"] / (g["Nr"] - 1.5)
     eo, io = C.species_config(name, {"dx": dx, "dr": dr, "dt": g["dt"]}, **over)
-    plasma = C.load_plasma(g, cfg["ppc"], cfg["dens"], seed=0, max_particles=max_particles)
+    plasma = C.load_plasma(g, cfg["ppc"], cfg["dens"], seed=0, max_particles=max_particles,
+                           order="cell")
     kx, kv = C.density_profile(name, g, **over)
     sim = DiffPIC(g, las, (eo, io), plasma, kx, comm, ckpt=ckpt or cfg.get("ckpt", 10))
     return sim, las["a0"], las["R"], np.asarray(kv, dtype=np.float64), cfg["steps"]

@@ -103,11 +103,15 @@ def n_particles(Nx, Nr, ppc):
     return (Nx - 3) * (Nr - 2) * int(np.prod(ppc))
 
 
-def load_plasma(grid_cfg, ppc, dens, seed=0, max_particles=None):
+def load_plasma(grid_cfg, ppc, dens, seed=0, max_particles=None, order="load"):
     """Seeded uniform plasma on the valid cells, weights w0*r (no profile).
 
     Returns a dict of float64 arrays x, y, z, px, py, pz, w, g_inv in the
-    particle order described in the module docstring."""
+    particle order described in the module docstring (order="load"), or
+    permuted stably into the order of the grid's cell keys (order="cell",
+    what Particles.align_parts would leave after a sort): the radial loading
+    cells [ir*dr, (ir+1)*dr) straddle two grid rows (Rmin = -dr/2), so in
+    load order each cell's particles form two runs far apart in memory."""
     Nx, Nr = grid_cfg["Nx"], grid_cfg["Nr"]
     dx = (grid_cfg["Xmax"] - grid_cfg["Xmin"]) / (Nx - 1)
     dr = grid_cfg["Rmax"] / (Nr - 1.5)
@@ -134,7 +138,25 @@ def load_plasma(grid_cfg, ppc, dens, seed=0, max_particles=None):
     out = {"x": x, "y": r * np.sin(th), "z": r * np.cos(th),
            "w": w0 * r, "px": np.zeros(Np), "py": np.zeros(Np),
            "pz": np.zeros(Np), "g_inv": np.ones(Np)}
+    if order == "cell":
+        perm = np.argsort(cell_keys(grid_cfg, out["x"], out["y"], out["z"]), kind="stable")
+        out = {k: np.ascontiguousarray(v[perm]) for k, v in out.items()}
+    elif order != "load":
+        raise ValueError(order)
     return out
+
+
+def cell_keys(grid_cfg, x, y, z):
+    """The sort key of index_and_sum_in_cell (particles_generic.cl:105-124)
+    in float64 with the Grid's derived geometry (grid.py:68-103)."""
+    Nx, Nr = grid_cfg["Nx"], grid_cfg["Nr"]
+    dx = (grid_cfg["Xmax"] - grid_cfg["Xmin"]) / (Nx - 1)
+    dr = grid_cfg["Rmax"] / (Nr - 1.5)
+    r = np.sqrt(y * y + z * z)
+    ix = np.floor((x - grid_cfg["Xmin"]) * (1.0 / dx)).astype(np.int64)
+    ir = np.floor((r - (-0.5 * dr)) * (1.0 / dr)).astype(np.int64)
+    ok = (ix > 0) & (ix < Nx - 2) & (ir >= 0) & (ir < Nr - 2)
+    return np.where(ok, ix + ir * (Nx - 1), (Nr - 1) * (Nx - 1))
 
 
 class ParticleSource:

@@ -90,24 +90,39 @@ __global__ void __launch_bounds__(64)
   for (int m = 0; m < MM; m++) ar[m] = ai[m] = 0;
   for (uint32_t cs = S0; cs < S1; cs += CAP) {
     const uint32_t n = (S1 - cs < CAP) ? S1 - cs : CAP;
-    for (uint32_t t = lane; t < n; t += 64) {
-      const uint32_t p = sort_indx[cs + t];
-      const uint32_t q = DEP_SW(t);
-      const T xp = x[p], yp = y[p], zp = z[p];
-      T wp, jp[3] = {0, 0, 0};
-      if (KIND == 2) {
-        jp[0] = ux[p];
-        jp[1] = uy[p];
-        jp[2] = uz[p];
-        wp = (T)1;
-      } else if (KIND == 1) {
-        jp[0] = ux[p];
-        jp[1] = uy[p];
-        jp[2] = uz[p];
-        wp = w[p] * g_inv[p] * charge;
-      } else {
-        wp = w[p] * charge;
+    // all loads of the chunk first (one latency), then the weights
+    constexpr int R = (CAP + 63) / 64;
+    T lx[R], ly[R], lz[R], lw[R], lu[R][3];
+#pragma unroll
+    for (int rr = 0; rr < R; rr++) {
+      const uint32_t t = lane + 64 * rr;
+      lx[rr] = ly[rr] = lz[rr] = lw[rr] = 0;
+      lu[rr][0] = lu[rr][1] = lu[rr][2] = 0;
+      if (t < n) {
+        const uint32_t p = sort_indx[cs + t];
+        lx[rr] = x[p];
+        ly[rr] = y[p];
+        lz[rr] = z[p];
+        if (KIND == 1) {
+          lw[rr] = w[p] * g_inv[p] * charge;
+        } else if (KIND == 0) {
+          lw[rr] = w[p] * charge;
+        } else {
+          lw[rr] = (T)1;
+        }
+        if (VEC) {
+          lu[rr][0] = ux[p];
+          lu[rr][1] = uy[p];
+          lu[rr][2] = uz[p];
+        }
       }
+    }
+#pragma unroll
+    for (int rr = 0; rr < R; rr++) {
+      const uint32_t t = lane + 64 * rr;
+      if (t >= n) continue;
+      const uint32_t q = DEP_SW(t);
+      const T xp = lx[rr], yp = ly[rr], zp = lz[rr], wp = lw[rr];
       const T rp = dsqrt<T>(yp * yp + zp * zp);
       T rp_inv;
       if (VEC) {
@@ -151,12 +166,13 @@ __global__ void __launch_bounds__(64)
       }
       if (VEC) {
 #pragma unroll
-        for (int kk = 0; kk < 3; kk++) st[VU + kk][q] = jp[kk];
+        for (int kk = 0; kk < 3; kk++) st[VU + kk][q] = lu[rr][kk];
       }
     }
     __syncthreads();
     if (act) {
       uint32_t lo = ip0 > cs ? ip0 : cs, hi = ip1 < cs + n ? ip1 : cs + n;
+#pragma unroll 4
       for (uint32_t ip = lo; ip < hi; ip++) {
         const uint32_t t = DEP_SW(ip - cs);
         const T C = st[corner][t];
@@ -601,9 +617,11 @@ __device__ __forceinline__ T field_val(const FieldPtrs &fp, int v, int64_t node)
 
 // E_p, B_p of one particle from its 4 nodes (value v of node (i, j) from
 // ld(v, i, j), values ordered as in GatherCfg), in k_gather_push's order.
-template <typename T, int M, typename LD>
+// DBL: the loader already returns the mode values doubled (exact).
+template <typename T, int M, bool DBL, typename LD>
 __device__ __forceinline__ void gather_sum(LD ld, const T (&C)[2][2], const T *gr, const T *gi,
                                            T *e_p, T *b_p) {
+  const T two = DBL ? (T)1 : (T)2;
   constexpr int NQ = 1 + 2 * M;
 #pragma unroll
   for (int k = 0; k < 3; k++)
@@ -615,8 +633,14 @@ __device__ __forceinline__ void gather_sum(LD ld, const T (&C)[2][2], const T *g
         b_p[k] += C[i][j] * ld(3 * NQ + k * NQ, i, j);
 #pragma unroll
         for (int m = 1; m <= M; m++) {
-          T er = 2 * ld(k * NQ + 2 * m - 1, i, j), ei = 2 * ld(k * NQ + 2 * m, i, j);
-          T br = 2 * ld(3 * NQ + k * NQ + 2 * m - 1, i, j), bi = 2 * ld(3 * NQ + k * NQ + 2 * m, i, j);
+          T er = ld(k * NQ + 2 * m - 1, i, j), ei = ld(k * NQ + 2 * m, i, j);
+          T br = ld(3 * NQ + k * NQ + 2 * m - 1, i, j), bi = ld(3 * NQ + k * NQ + 2 * m, i, j);
+          if (!DBL) {
+            er = two * er;
+            ei = two * ei;
+            br = two * br;
+            bi = two * bi;
+          }
           e_p[k] += C[i][j] * er * gr[m];
           e_p[k] -= C[i][j] * ei * gi[m];
           b_p[k] += C[i][j] * br * gr[m];
@@ -645,12 +669,16 @@ __global__ void __launch_bounds__(GS_NT)
   const int64_t nlive = (np_stay - s0 < GS_NT) ? np_stay - s0 : GS_NT;
   uint32_t p = 0;
   T xp = 0, yp = 0, zp = 0, rp = 0;
+  T u[3] = {0, 0, 0};
   int ix = 0, ir = 0;
   if (live) {
     p = sort_indx[slot];
     xp = x[p];
     yp = y[p];
     zp = z[p];
+    u[0] = px[p];
+    u[1] = py[p];
+    u[2] = pz[p];
     rp = dsqrt<T>(yp * yp + zp * zp);
     ix = (int)dfloor<T>((xp - xmin) * dx_inv);
     ir = (int)dfloor<T>((rp - rmin) * dr_inv);
@@ -670,14 +698,14 @@ __global__ void __launch_bounds__(GS_NT)
       const int64_t cc = cA + cl;
       const int irc = (int)(cc / Nx_cell), ixc = (int)(cc - (int64_t)irc * Nx_cell);
       const int64_t node = ixc + (corner & 1) + (int64_t)(irc + (corner >> 1)) * Nx;
-      fs[(cl * 4 + corner) * NV + v] = field_val<T, M>(fp, v, node);
+      const T val = field_val<T, M>(fp, v, node);
+      fs[(cl * 4 + corner) * NV + v] = (v % NQ) ? (T)2 * val : val;
     }
   }
   __syncthreads();
   if (!live || !((int64_t)p < np_stay)) return;
   const int64_t igrid = ix + (int64_t)ir * Nx;
   const int lc = (int)(cell - cA);
-  T u[3] = {px[p], py[p], pz[p]};
   T dt_2 = (T)0.5 * factor_push;
   T rp_inv = (T)1. / rp;
   T sX1 = (xp - xmin) * dx_inv - (T)ix;
@@ -700,10 +728,10 @@ __global__ void __launch_bounds__(GS_NT)
   T e_p[3] = {0, 0, 0}, b_p[3] = {0, 0, 0};
   if (staged) {
     const T *f0 = fs + lc * 4 * NV;
-    gather_sum<T, M>([&](int v, int i, int j) { return f0[(2 * i + j) * NV + v]; }, C, gr, gi,
-                     e_p, b_p);
+    gather_sum<T, M, true>([&](int v, int i, int j) { return f0[(2 * i + j) * NV + v]; }, C, gr,
+                           gi, e_p, b_p);
   } else {
-    gather_sum<T, M>([&](int v, int i, int j) {
+    gather_sum<T, M, false>([&](int v, int i, int j) {
       return field_val<T, M>(fp, v, igrid + j + (int64_t)Nx * i);
     }, C, gr, gi, e_p, b_p);
   }

@@ -96,8 +96,21 @@ __device__ __forceinline__ unsigned long long wave_runs(uint32_t rkey, int lane,
   return heads;
 }
 
+// Wave-aggregated increment: lanes with act and equal consecutive values
+// v form runs; each run's head adds its length to slot[v] with one atomic.
+__device__ __forceinline__ void run_add(uint32_t *slot_base, bool act, uint32_t v, int lane) {
+  const uint32_t rv = act ? v : (0xFFFFFFFFu - (uint32_t)lane);
+  int len, hl;
+  wave_runs(rv, lane, &len, &hl);
+  if (act && hl == lane) atomicAdd(&slot_base[v], (uint32_t)len);
+}
+
 // With PUSH, the coordinate push of push_xyz (same rounding order) is applied
-// first and the key is taken from the pushed position.
+// first and the key is taken from the pushed position.  Besides the keys and
+// the cell histogram it adds up, per RS_TILE tile of particles, the counts
+// of the sort's first radix digit (nb0 bits; integer atomics, aggregated
+// over runs of equal digits in a wave), so the sort's first pass needs no
+// separate counting sweep.
 template <typename T, bool PUSH>
 __global__ void __launch_bounds__(SORT_BS) k_cell_key(T *__restrict__ x, T *__restrict__ y,
                                                       T *__restrict__ z, const T *__restrict__ px,
@@ -105,12 +118,13 @@ __global__ void __launch_bounds__(SORT_BS) k_cell_key(T *__restrict__ x, T *__re
                                                       const T *__restrict__ g_inv, T dt, int64_t Np,
                                                       int Nx, int Nr, T xmin, T dx_inv, T rmin, T dr_inv,
                                                       uint32_t *__restrict__ key_out,
-                                                      uint32_t *__restrict__ hist) {
+                                                      uint32_t *__restrict__ hist, int nb0,
+                                                      uint32_t *__restrict__ counts, int64_t nblk) {
   const int Nx_loc = Nx - 1, Nr_loc = Nr - 1;
   const uint32_t ovf = (uint32_t)(Nr_loc * Nx_loc);
   const int lane = threadIdx.x & 63;
-  int64_t ip = (int64_t)blockIdx.x * SORT_BS + threadIdx.x;
-  bool act = ip < Np;
+  const int64_t ip = (int64_t)blockIdx.x * SORT_BS + threadIdx.x;
+  const bool act = ip < Np;
   uint32_t key = ovf;
   if (act) {
     T xp = x[ip], yp = y[ip], zp = z[ip];
@@ -129,11 +143,11 @@ __global__ void __launch_bounds__(SORT_BS) k_cell_key(T *__restrict__ x, T *__re
     key = cell_key<T>(xp, yp, zp, Nx_loc, Nr_loc, xmin, dx_inv, rmin, dr_inv);
     key_out[ip] = key;
   }
-  bool valid = act && key != ovf;
-  uint32_t rkey = valid ? key : (0xFFFFFFFFu - (uint32_t)lane);
-  int len, hl;
-  wave_runs(rkey, lane, &len, &hl);
-  if (valid && hl == lane) atomicAdd(&hist[key], (uint32_t)len);
+  run_add(hist, act && key != ovf, key, lane);
+  // a wave lies inside one tile (RS_TILE is a multiple of 64)
+  const int64_t tile = ((int64_t)blockIdx.x * SORT_BS) / RS_TILE;
+  const uint32_t d = key & (((uint32_t)1 << nb0) - 1u);
+  run_add(counts + tile, act, d * (uint32_t)nblk, lane);
 }
 
 __global__ void k_zero_u32(uint32_t *__restrict__ a, int64_t n) {
@@ -178,22 +192,16 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_reduce(const uint32_t *__restr
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(SCAN_BS) k_scan_blocks(uint32_t *__restrict__ bsum, int64_t nb) {
-  uint32_t carry = 0;
-  for (int64_t base = 0; base < nb; base += SCAN_BS) {
-    int64_t k = base + threadIdx.x;
-    uint32_t v = k < nb ? bsum[k] : 0;
-    uint32_t tot;
-    uint32_t ex = block_excl_scan(v, &tot);
-    if (k < nb) bsum[k] = carry + ex;
-    carry += tot;
-  }
-}
-
 // Exclusive scan of `in` (n entries) into out[0..n]; zeroes in[0..nzero).
+// bsum holds the tile sums of k_scan_reduce; each block adds up those of
+// the tiles before it.
 __global__ void __launch_bounds__(SCAN_BS) k_scan_final(uint32_t *__restrict__ in, int64_t n,
                                                         int64_t nzero, const uint32_t *__restrict__ bsum,
                                                         uint32_t *__restrict__ out) {
+  uint32_t bs = 0;
+  for (int64_t i = threadIdx.x; i < (int64_t)blockIdx.x; i += SCAN_BS) bs += bsum[i];
+  uint32_t bpre;
+  block_excl_scan(bs, &bpre);
   int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
   uint32_t v[SCAN_ITEMS];
   uint32_t s = 0;
@@ -203,7 +211,7 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_final(uint32_t *__restrict__ i
     s += v[i];
   }
   uint32_t tot;
-  uint32_t pre = block_excl_scan(s, &tot) + bsum[blockIdx.x];
+  uint32_t pre = block_excl_scan(s, &tot) + bpre;
   for (int i = 0; i < SCAN_ITEMS; i++) {
     int64_t k = base + i;
     if (k < n) {
@@ -239,10 +247,18 @@ __global__ void __launch_bounds__(RS_BS) k_rs_upsweep(const uint32_t *__restrict
   for (int d = threadIdx.x; d < nd; d += RS_BS) h[d] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-#pragma unroll 4
+  const int lane = threadIdx.x & 63;
+  uint32_t kk[RS_ITEMS];
+#pragma unroll
   for (int i = 0; i < RS_ITEMS; i++) {
     int64_t idx = base + (int64_t)i * RS_BS + threadIdx.x;
-    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
+    kk[i] = idx < n ? keys[idx] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < RS_ITEMS; i++) {
+    int64_t idx = base + (int64_t)i * RS_BS + threadIdx.x;
+    const bool act = idx < n;
+    run_add(h, act, (kk[i] >> shift) & mask, lane);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < nd; d += RS_BS) counts[(int64_t)d * nblk + blockIdx.x] = h[d];
@@ -273,17 +289,43 @@ __global__ void __launch_bounds__(RS_BS) k_rs_downsweep(const uint32_t *__restri
   __syncthreads();
   const unsigned long long lt = (1ull << lane) - 1ull;
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  uint32_t kk[RS_ITEMS], vv[RS_ITEMS];
+#pragma unroll
   for (int r = 0; r < RS_ITEMS; r++) {
     const int64_t idx = base + (int64_t)r * RS_BS + tid;
     const bool act = idx < n;
-    uint32_t k = act ? kin[idx] : 0u;
-    uint32_t v = act ? (vin ? vin[idx] : (uint32_t)idx) : 0u;
+    kk[r] = act ? kin[idx] : 0u;
+    vv[r] = act ? (vin ? vin[idx] : (uint32_t)idx) : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; r++) {
+    const int64_t idx = base + (int64_t)r * RS_BS + tid;
+    const bool act = idx < n;
+    const uint32_t k = kk[r], v = vv[r];
     uint32_t d = (k >> shift) & mask;
-    unsigned long long peers = __ballot(act);
-    for (int b = 0; b < nbits; b++) {
-      bool bit = (d >> b) & 1u;
-      unsigned long long bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
+    const unsigned long long amask = __ballot(act);
+    unsigned long long peers;
+    // active lanes are a prefix of the wave; lane 0 is active if any is
+    const uint32_t d0 = (uint32_t)__shfl((int)d, 0);
+    const uint32_t dprev = (uint32_t)__shfl_up((int)d, 1);
+    if (__ballot(act && d == d0) == amask) {
+      peers = amask;  // one digit in the wave
+    } else if (__ballot(act && lane > 0 && dprev > d) == 0ull) {
+      // non-decreasing digits: equal digits form contiguous runs
+      const unsigned long long heads = __ballot(act && (lane == 0 || dprev != d));
+      const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+      const int hl = 63 - __clzll((long long)(heads & upto));
+      const unsigned long long above = heads & ~upto;
+      const int nx = above ? (__ffsll((long long)above) - 1) : 64;
+      const unsigned long long hi = (nx == 64) ? ~0ull : ((1ull << nx) - 1ull);
+      peers = hi & ~((1ull << hl) - 1ull) & amask;
+    } else {
+      peers = amask;
+      for (int b = 0; b < nbits; b++) {
+        bool bit = (d >> b) & 1u;
+        unsigned long long bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+      }
     }
     const uint32_t rank = (uint32_t)__popcll((long long)(peers & lt));
     const uint32_t cnt = (uint32_t)__popcll((long long)peers);
@@ -392,11 +434,14 @@ static int sort_keys(T *x, T *y, T *z, const T *px, const T *py, const T *pz, co
   SortWs w;
   if (sort_layout(Np, nbins, &w, ws) > ws_bytes || !ws) return CHM_EWORKSPACE;
   hipStream_t st = (hipStream_t)stream;
+  const int nb0 = rs_plan(key_bits(nbins)).nb[0];
+  const int64_t nc0 = ((int64_t)1 << nb0) * w.nblk;
   k_zero_u32<<<chm_blocks(nbins, 256), 256, 0, st>>>(w.hist, nbins);
+  k_zero_u32<<<chm_blocks(nc0, 256), 256, 0, st>>>(w.counts, nc0);
   if (Np > 0)
     k_cell_key<T, PUSH><<<chm_blocks(Np, SORT_BS), SORT_BS, 0, st>>>(
         x, y, z, px, py, pz, g_inv, dt, Np, g->Nx, g->Nr, (T)g->xmin, (T)g->dx_inv, (T)g->rmin,
-        (T)g->dr_inv, indx_in_cell, w.hist);
+        (T)g->dr_inv, indx_in_cell, w.hist, nb0, w.counts, w.nblk);
   CHM_CHECK_LAUNCH();
   return CHM_OK;
 }
@@ -404,7 +449,6 @@ static int sort_keys(T *x, T *y, T *z, const T *px, const T *py, const T *pz, co
 static void scan_u32(const uint32_t *in, int64_t n, uint32_t *out, uint32_t *bsum, int64_t nscan,
                      hipStream_t st) {
   k_scan_reduce<<<(unsigned)nscan, SCAN_BS, 0, st>>>(in, n, bsum);
-  k_scan_blocks<<<1, SCAN_BS, 0, st>>>(bsum, nscan);
   k_scan_final<<<(unsigned)nscan, SCAN_BS, 0, st>>>((uint32_t *)in, n, 0, bsum, out);
 }
 
@@ -434,10 +478,13 @@ static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_
     uint32_t *vout = last ? sort_indx : ((p & 1) ? w.vB : w.vA);
     const int64_t nc = ((int64_t)1 << nb) * w.nblk;
     const int64_t nsc = (nc + SCAN_TILE - 1) / SCAN_TILE;
-    if (plan.rb == 8)
-      k_rs_upsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
-    else
-      k_rs_upsweep<11><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
+    // the first digit's tile counts come from the key kernel
+    if (p > 0) {
+      if (plan.rb == 8)
+        k_rs_upsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
+      else
+        k_rs_upsweep<11><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
+    }
     scan_u32(w.counts, nc, w.offs, w.bsum2, nsc, st);
     if (plan.rb == 8)
       k_rs_downsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs, w.nblk,

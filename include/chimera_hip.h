@@ -180,7 +180,8 @@ int chm_index_sort_f32(const float *x, const float *y, const float *z,
 /* The two halves of index_sort, for the step's push-then-sort sequence
  * (pic_loop.py:70-91, push_coords followed by sort_parts):
  * chm_push_x_keys_* applies the coordinate push of chm_push_x_* and writes
- * the cell keys and the histogram into the sort workspace in the same pass;
+ * the cell keys, the cell histogram and the per-tile counts of the sort's
+ * first radix digit into the sort workspace in the same pass;
  * chm_sort_from_keys then produces cell_offset, sort_indx and np_stay
  * exactly as chm_index_sort_* does. */
 int chm_push_x_keys_f64(double *x, double *y, double *z, const double *px,

@@ -47,3 +47,17 @@ def test_density_profiles():
     x, f = C.density_profile("C4", C.grid_config("C4"))
     assert len(x) == 32 and np.all(np.abs(np.array(f) - 1) <= 0.2)
     assert C.density_profile("C1", C.grid_config("C1")) is None
+
+
+def test_cell_order_is_a_stable_permutation_into_key_order():
+    g = C.grid_config("C1", Nx=40, Nr=12)
+    a = C.load_plasma(g, (2, 2, 4), 0.002, seed=0)
+    b = C.load_plasma(g, (2, 2, 4), 0.002, seed=0, order="cell")
+    ka = C.cell_keys(g, a["x"], a["y"], a["z"])
+    kb = C.cell_keys(g, b["x"], b["y"], b["z"])
+    assert np.all(np.diff(kb) >= 0)
+    perm = np.argsort(ka, kind="stable")
+    for k in a:
+        assert np.array_equal(a[k][perm], b[k])
+    # every load cell straddles two grid rows: the load order is not key order
+    assert np.any(np.diff(ka) < 0)
