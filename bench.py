@@ -277,6 +277,21 @@ def kernel_table(sim, reps, w):
     return rows
 
 
+def pmc_traffic(config, form):
+    """HBM bytes per gather launch measured with rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE (copy-calibrated) on this workload, from profiles/."""
+    if config != "C2":
+        return None
+    path = os.path.join(ROOT, "profiles", "gather_push_c2_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes:"):
+                return json.loads(line.split(":", 1)[1]).get(form)
+    except OSError:
+        return None
+    return None
+
+
 def gather_roofline(sim, reps, w):
     """The dominant kernel (gather + Boris push), timed with HIP events on the
     stream it runs on.  Algorithmic bytes per launch: each electron reads x,
@@ -489,6 +504,7 @@ def main():
     ms = dt / args.steps * 1e3
     pps = Ne * args.steps / dt
     roof = gather_roofline(fp, args.kernel_reps, 8)
+    roof["traffic"] = pmc_traffic(args.config, "f64")
     mixed = None
     if mx is not None:
         dtm = timing["mixed"]
@@ -496,6 +512,7 @@ def main():
                  "ms_per_step": dtm / args.steps * 1e3, "pic_steps_per_s": args.steps / dtm,
                  "speedup_vs_f64": dt / dtm, "dtype": "f32 particles/FFT, split-bf16x3 DHT",
                  "gather_roofline": gather_roofline(mx, args.kernel_reps, 4)}
+        mixed["gather_roofline"]["traffic"] = pmc_traffic(args.config, "mixed")
     kernels = {"f64": kernel_table(fp, args.kernel_reps, 8)}
     if mx is not None:
         kernels["mixed"] = kernel_table(mx, args.kernel_reps, 4)
