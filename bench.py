@@ -313,6 +313,39 @@ def gather_roofline(sim, reps, w):
             "avg_launch_us": t * 1e6, "algorithmic_bytes": nbytes}
 
 
+def moving_window(sim, events=4, steps_per_event=20):
+    """Frame (frame.py:4-64) on the FP64 run: window shifts of 20 cells at
+    c with plasma injection on the GPU, sort and align of both species,
+    every 20 steps; times the frame events and the eager steps between."""
+    import torch
+    import chimeracl_amd as CA
+    s = sim.solver
+    frame = CA.Frame({"Velocity": 1.0, "dt": s.Args["dt"], "Steps": steps_per_event})
+    sim.ions.Args["InjectorSource"] = sim.eons
+    loop = CA.PIC_loop(solvers=[s], species=[sim.eons, sim.ions], frames=[frame])
+    loop.it = 1
+    t_ev = t_all = 0.0
+    n0 = sim.eons.Args["Np"]
+    for e in range(events):
+        torch.cuda.synchronize()
+        t0 = time.time()
+        frame.shift_grids(grids=[s])
+        frame.inject_plasma(species=[sim.eons, sim.ions], grid=s)
+        torch.cuda.synchronize()
+        t1 = time.time()
+        for _ in range(steps_per_event - 1):
+            loop.step()
+        torch.cuda.synchronize()
+        t_ev += t1 - t0
+        t_all += time.time() - t0
+    nsteps = events * steps_per_event
+    return {"events": events, "steps": nsteps, "ms_per_event": t_ev / events * 1e3,
+            "ms_per_step_with_window": t_all / nsteps * 1e3,
+            "particle_steps_per_s": sim.eons.Args["Np"] * nsteps / t_all,
+            "electrons_before": n0, "electrons_after": sim.eons.Args["Np"],
+            "note": "eager steps (a window step is not graph-captured)"}
+
+
 # ---------------------------------------------------------------- differentiable form
 def diff_section():
     import torch
@@ -516,6 +549,7 @@ def main():
     kernels = {"f64": kernel_table(fp, args.kernel_reps, 8)}
     if mx is not None:
         kernels["mixed"] = kernel_table(mx, args.kernel_reps, 4)
+    window = moving_window(fp)
     del fp, mx, sims
     gc.collect()
     torch.cuda.empty_cache()
@@ -547,6 +581,7 @@ def main():
         "mixed": mixed,
         "diagnostics": diags,
         "kernels": kernels,
+        "moving_window": window,
         "differentiable": diff,
     }
     print(json.dumps(out), flush=True)

@@ -24,6 +24,9 @@ def __getattr__(name):
     if name == "add_gausian_pulse":
         from .laser import add_gausian_pulse
         return add_gausian_pulse
+    if name == "Frame":
+        from .frame import Frame
+        return Frame
     if name == "Communicator":
         from .methods.generic_methods_hip import Communicator
         return Communicator

@@ -100,6 +100,11 @@ _SIGS = {
     "chm_convert_table": [P, i64, P, P, P, P],
     "chm_laser_pulse_f64": [P, i32, i32, f64, f64, f64, f64, f64, f64, f64, f64, f64, P],
     "chm_laser_spectral_f64": [R(Spectral), P, P, P, f64, P],
+    "chm_fill_domain_f64": [P, P, P, P, P, P, P, P, P, P, i32, i64, i32, i32, i32, f64, f64, f64,
+                            f64, ctypes.c_uint64, ctypes.c_uint64, P],
+    "chm_fill_domain_f32": [P, P, P, P, P, P, P, P, P, P, i32, i64, i32, i32, i32, f64, f64, f64,
+                            f64, ctypes.c_uint64, ctypes.c_uint64, P],
+    "chm_phase_tables_f64": [P, i32, f64, P, P, P],
     # adjoints (differentiable form)
     "chm_push_x_adj_f64": [P, P, P, P, P, P, P, P, P, P, P, f64, i64, P],
     "chm_push_x_adj_f32": [P, P, P, P, P, P, P, P, P, P, P, f32, i64, P],

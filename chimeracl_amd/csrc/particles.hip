@@ -588,3 +588,144 @@ extern "C" int chm_dens_profile_f32(const float *x, float *w, int64_t Np, const 
                                     void *stream) {
   return dens_profile<float>(x, w, Np, xx, ff, dxm1, nknots, stream);
 }
+
+// ------------------------------------------------------------------ domain fill
+// Deterministic building blocks shared with the CPU oracle (which evaluates
+// the same expressions in the same order, so the results are bit-identical):
+//   rng_u01   splitmix64 (Steele, Lea, Flood 2014) of seed + (counter+1)*golden,
+//             top 53 bits -> [0, 1)
+//   det_sincos sin and cos from IEEE +, -, *, / only: Cody-Waite reduction by
+//             pi/2 with the three-part constant of fdlibm (k*P1, k*P2 exact for
+//             |k| < 2^20), Taylor polynomials to r^17 / r^18 on |r| <= pi/4.
+__device__ __forceinline__ double rng_u01(uint64_t seed, uint64_t counter) {
+  uint64_t z = seed + (counter + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ void det_sincos(double a, double *s_out, double *c_out) {
+  const double P1 = 1.57079632673412561417e+00, P2 = 6.07710050630396597660e-11,
+               P2T = 2.02226624879595063154e-21;
+  const double k = floor(a * (2.0 / 3.14159265358979323846) + 0.5);
+  const double r = ((a - k * P1) - k * P2) - k * P2T;
+  const double r2 = r * r;
+  double s = 1.0 / 355687428096000.0;                 // 1/17!
+  s = (-1.0 / 1307674368000.0) + r2 * s;              // 1/15!
+  s = (1.0 / 6227020800.0) + r2 * s;                  // 1/13!
+  s = (-1.0 / 39916800.0) + r2 * s;                   // 1/11!
+  s = (1.0 / 362880.0) + r2 * s;                      // 1/9!
+  s = (-1.0 / 5040.0) + r2 * s;                       // 1/7!
+  s = (1.0 / 120.0) + r2 * s;                         // 1/5!
+  s = (-1.0 / 6.0) + r2 * s;                          // 1/3!
+  s = r + r * (r2 * s);
+  double c = -1.0 / 6402373705728000.0;               // -1/18!
+  c = (1.0 / 20922789888000.0) + r2 * c;              // 1/16!
+  c = (-1.0 / 87178291200.0) + r2 * c;                // -1/14!
+  c = (1.0 / 479001600.0) + r2 * c;                   // 1/12!
+  c = (-1.0 / 3628800.0) + r2 * c;                    // -1/10!
+  c = (1.0 / 40320.0) + r2 * c;                       // 1/8!
+  c = (-1.0 / 720.0) + r2 * c;                        // -1/6!
+  c = (1.0 / 24.0) + r2 * c;                          // 1/4!
+  c = (-0.5) + r2 * c;                                // -1/2!
+  c = 1.0 + r2 * c;
+  const int q = (int)((int64_t)k & 3);
+  double sv, cv;
+  if (q == 0) { sv = s; cv = c; }
+  else if (q == 1) { sv = c; cv = -s; }
+  else if (q == 2) { sv = -s; cv = -c; }
+  else { sv = -c; cv = s; }
+  *s_out = sv;
+  *c_out = cv;
+}
+
+// fill_grid, particles_generic.cl:33-84: per cell (ix, ir) of the local
+// grids (xgrid [Nx_loc], rgrid), Nppc_th x Nppc_r x Nppc_x particles at the
+// regular sub-positions, the theta set rotated by a per-cell uniform offset
+// in [0, 2 pi) (theta_variator, particles_methods_cl.py:95-98, here from
+// rng_u01(seed, counter0 + cell)); weights r * w0 (:103); momenta p_c and
+// g_inv (:111-140, cold or drifting plasma).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_fill_domain(T *__restrict__ x, T *__restrict__ y, T *__restrict__ z, T *__restrict__ w,
+                  T *__restrict__ px, T *__restrict__ py, T *__restrict__ pz,
+                  T *__restrict__ g_inv, const double *__restrict__ xgrid,
+                  const double *__restrict__ rgrid, int Nx, int64_t ncells, int npx, int npr,
+                  int npt, double w0, double pxc, double pyc, double pzc, int moving,
+                  uint64_t seed, uint64_t counter0) {
+  const int64_t i_cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i_cell >= ncells) return;
+  const int Nx_cell = Nx - 1;
+  const int ir = (int)(i_cell / Nx_cell), ix = (int)(i_cell - (int64_t)ir * Nx_cell);
+  int64_t ip = i_cell * (int64_t)(npx * npr * npt);
+  const double xmin = xgrid[ix], rmin = rgrid[ir];
+  const double thmin = (2.0 * 3.14159265358979323846) * rng_u01(seed, counter0 + (uint64_t)i_cell);
+  const double Lx = xgrid[ix + 1] - xgrid[ix];
+  const double Lr = rgrid[ir + 1] - rgrid[ir];
+  const double dx = 1. / ((double)npx);
+  const double dr = 1. / ((double)npr);
+  const double dth = 2 * 3.14159265358979323846 / ((double)npt);
+  double g = 1.0;
+  if (pxc != 0.0 || pyc != 0.0 || pzc != 0.0) g = 1. / sqrt(1 + pxc * pxc + pyc * pyc + pzc * pzc);
+  for (int it = 0; it < npt; it++) {
+    const double th = thmin + it * dth;
+    double sin_th, cos_th;
+    det_sincos(th, &sin_th, &cos_th);
+    for (int irr = 0; irr < npr; irr++) {
+      const double rp = rmin + (0.5 + irr) * dr * Lr;
+      const double rp_s = rp * sin_th;
+      const double rp_c = rp * cos_th;
+      for (int ixx = 0; ixx < npx; ixx++) {
+        x[ip] = (T)(xmin + (0.5 + ixx) * dx * Lx);
+        y[ip] = (T)rp_s;
+        z[ip] = (T)rp_c;
+        w[ip] = (T)(rp * w0);
+        if (moving) {
+          px[ip] = (T)pxc;
+          py[ip] = (T)pyc;
+          pz[ip] = (T)pzc;
+          g_inv[ip] = (T)g;
+        }
+        ip += 1;
+      }
+    }
+  }
+}
+
+template <typename T>
+static int fill_domain(T *x, T *y, T *z, T *w, T *px, T *py, T *pz, T *g_inv, const double *xgrid,
+                       const double *rgrid, int32_t Nx_loc, int64_t ncells, int32_t npx,
+                       int32_t npr, int32_t npt, double w0, double pxc, double pyc, double pzc,
+                       uint64_t seed, uint64_t counter0, void *stream) {
+  if (!x || !y || !z || !w || !xgrid || !rgrid || Nx_loc < 2 || ncells < 0 || npx < 1 || npr < 1 ||
+      npt < 1)
+    return CHM_EINVAL;
+  const int moving = px != nullptr;
+  if (moving && (!py || !pz || !g_inv)) return CHM_EINVAL;
+  if (ncells == 0) return CHM_OK;
+  k_fill_domain<T><<<chm_blocks(ncells, 256), 256, 0, (hipStream_t)stream>>>(
+      x, y, z, w, px, py, pz, g_inv, xgrid, rgrid, Nx_loc, ncells, npx, npr, npt, w0, pxc, pyc, pzc,
+      moving, seed, counter0);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_fill_domain_f64(double *x, double *y, double *z, double *w, double *px,
+                                   double *py, double *pz, double *g_inv, const double *xgrid,
+                                   const double *rgrid, int32_t Nx_loc, int64_t ncells,
+                                   int32_t npx, int32_t npr, int32_t npt, double w0, double pxc,
+                                   double pyc, double pzc, uint64_t seed, uint64_t counter0,
+                                   void *stream) {
+  return fill_domain<double>(x, y, z, w, px, py, pz, g_inv, xgrid, rgrid, Nx_loc, ncells, npx, npr,
+                             npt, w0, pxc, pyc, pzc, seed, counter0, stream);
+}
+extern "C" int chm_fill_domain_f32(float *x, float *y, float *z, float *w, float *px, float *py,
+                                   float *pz, float *g_inv, const double *xgrid,
+                                   const double *rgrid, int32_t Nx_loc, int64_t ncells,
+                                   int32_t npx, int32_t npr, int32_t npt, double w0, double pxc,
+                                   double pyc, double pzc, uint64_t seed, uint64_t counter0,
+                                   void *stream) {
+  return fill_domain<float>(x, y, z, w, px, py, pz, g_inv, xgrid, rgrid, Nx_loc, ncells, npx, npr,
+                            npt, w0, pxc, pyc, pzc, seed, counter0, stream);
+}

@@ -253,3 +253,24 @@ extern "C" int chm_convert_table(const double *src, int64_t n, float *f32, uint1
   CHM_CHECK_LAUNCH();
   return CHM_OK;
 }
+
+// ---------------------------------------------------------------- frame
+// get_phase_minus / get_phase_plus (transformer_generic.cl:28-55) for a new
+// window origin xmin (moving window, frame.py:22-30).
+__global__ void k_phases(const double *kx, int Nx, double xmin, double2 *pm, double2 *pp) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Nx) return;
+  double a = xmin * kx[i];
+  double c = cos(a), s = sin(a);
+  pm[i] = make_double2(c, -s);
+  pp[i] = make_double2(c, s);
+}
+
+extern "C" int chm_phase_tables_f64(const double *kx, int32_t Nx, double xmin, void *phase_minus,
+                                    void *phase_plus, void *stream) {
+  if (!kx || !phase_minus || !phase_plus || Nx < 1) return CHM_EINVAL;
+  k_phases<<<chm_blocks(Nx, 256), 256, 0, (hipStream_t)stream>>>(kx, Nx, xmin, (double2 *)phase_minus,
+                                                                 (double2 *)phase_plus);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}

@@ -43,6 +43,13 @@ class GridMethodsHIP(GenericMethodsHIP):
             g = self._pcache["grid"] = _grid_struct(self)
         return g
 
+    def _frame_moved(self):
+        """After Frame.shift_grids: rebuild the cached geometry (and, for a
+        solver, the transform phases of the new window origin)."""
+        self._pcache.pop("grid", None)
+        if hasattr(self, "_refresh_phases"):
+            self._refresh_phases()
+
     def _dep_ws(self):
         ws = self._pcache.get("dep_ws")
         if ws is None:

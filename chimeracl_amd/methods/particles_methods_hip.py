@@ -51,9 +51,51 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         self.version += 1
 
     def make_new_domain(self, parts_in, density_profiles=None):
-        raise NotImplementedError(
-            "device-side random plasma loading is not part of this build; load particles with "
-            "chimeracl_amd.config.load_plasma and add_particles(source=...)")
+        """particles_methods_cl.py:66-147: a plasma block on the local grids of
+        parts_in (Xmin, Xmax, Rmin, Rmax) into DataDev['<c>_new'], filled on the
+        GPU (fill_grid, particles_generic.cl:33-84).  The per-cell theta offsets
+        come from a counter-based generator (Args 'seed', advancing
+        Args['rng_counter']) instead of PyOpenCL's.  Cold or drifting plasma
+        (px_c, py_c, pz_c); a thermal spread (dpx, ...) is not supported."""
+        a = self.Args
+        xmin, xmax, rmin, rmax = (parts_in[k] for k in ("Xmin", "Xmax", "Rmin", "Rmax"))
+        Nx_loc = int(np.ceil((xmax - xmin) / a["dx"]) + 1)
+        Nr_loc = int(np.round((rmax - rmin) / a["dr"]) + 1)
+        xg = xmin + a["dx"] * np.arange(Nx_loc)
+        rg = rmin + a["dr"] * np.arange(Nr_loc)
+        a["right_lim"] = float(xg[-1])
+        ncells = (Nx_loc - 1) * (Nr_loc - 1)
+        npx, npr, npt = (int(v) for v in a["Nppc"])
+        Np = ncells * npx * npr * npt
+        dev, real = self.comm.device, self.comm.real
+        new = {k: torch.empty(Np, dtype=real, device=dev) for k in ("x", "y", "z", "w")}
+        moving = "Immobile" not in a
+        pc = [0.0, 0.0, 0.0]
+        if moving:
+            for i, k in enumerate(("px", "py", "pz")):
+                if parts_in.get("d" + k, 0):
+                    raise NotImplementedError(
+                        "thermal momentum spreads (d%s) need a normal generator; only cold or "
+                        "drifting plasma is filled on the GPU" % k)
+                pc[i] = float(parts_in.get(k + "_c", 0.0))
+            for k in ("px", "py", "pz", "g_inv"):
+                new[k] = torch.empty(Np, dtype=real, device=dev)
+        grids = [torch.from_numpy(v).to(dev) for v in (xg, rg)]
+        counter = int(a.get("rng_counter", 0))
+        p = lambda k: ptr(new[k]) if k in new else None
+        self.call("chm_fill_domain", p("x"), p("y"), p("z"), p("w"), p("px"), p("py"), p("pz"),
+                  p("g_inv"), ptr(grids[0]), ptr(grids[1]), Nx_loc, ncells, npx, npr, npt,
+                  float(a["w0"]), pc[0], pc[1], pc[2], int(a.get("seed", 0)), counter)
+        a["rng_counter"] = counter + ncells
+        for k, t in new.items():
+            self.DataDev[k + "_new"] = t
+        if density_profiles is not None:
+            for profile in density_profiles:
+                if profile["coord"] != "x":
+                    continue  # only longitudinal profiles, as in the reference
+                self.dens_profile(profile["points"], profile["values"], parts_in["Xmin"],
+                                  parts_in["Xmax"], coord=profile["coord"] + "_new",
+                                  weight="w_new")
 
     def make_new_beam(self, parts_in):
         raise NotImplementedError(
@@ -96,8 +138,9 @@ class ParticleMethodsHIP(GenericMethodsHIP):
             self._keys_for = None
         else:
             self._ensure_sort_buffers(grid)
-            _, ws, gs = self._sort_ws
-            self.call("chm_push_x_keys", *args, gs, ptr(d["indx_in_cell"]), ptr(ws), ws.numel())
+            _, ws, _ = self._sort_ws
+            self.call("chm_push_x_keys", *args, grid._grid(), ptr(d["indx_in_cell"]), ptr(ws),
+                      ws.numel())
             self._keys_for = grid
         self.flag_sorted = False
         self.version += 1
@@ -116,14 +159,15 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         self.DataDev["cell_offset"] = torch.empty(nbins + 1, dtype=torch.int32, device=dev)
         self.DataDev["Np_stay"] = torch.zeros(1, dtype=torch.int32, device=dev)
         self.DataDev["sum_in_cell"] = ws[: nbins * 4].view(torch.int32)
-        self._sort_ws = (key, ws, _grid_struct(grid))
+        self._sort_ws = (key, ws, None)
 
     def index_sort(self, grid):
         """particles_methods_cl.py:225-261: cell keys, histogram, offsets and
         the stable in-cell order; Np_stay stays on the device and is read
         lazily from Args."""
         self._ensure_sort_buffers(grid)
-        _, ws, gs = self._sort_ws
+        _, ws, _ = self._sort_ws
+        gs = grid._grid()
         d = self.DataDev
         if self._keys_for is grid:
             _lib.call("chm_sort_from_keys", self.Args["Np"], gs, ptr(d["indx_in_cell"]),

@@ -129,6 +129,16 @@ class TransformerMethodsHIP(GenericMethodsHIP):
                   _lib.lib.chm_spectral_workspace_bytes(Nx, Nr, M, es))
         self._tws = torch.empty(nws, dtype=torch.uint8, device=dev)
 
+    def _refresh_phases(self):
+        """get_phase_minus/plus (transformer_generic.cl:28-55) for the current
+        Xmin; the reference recomputes them in every transform call."""
+        a, D = self.Args, self.DataDev
+        _lib.call("chm_phase_tables_f64", ptr(D["kx"]), a["Nx"], float(a["Xmin"]), ptr(D["phase_minus"]),
+                  ptr(D["phase_plus"]), self.comm.stream())
+        if self.comm.mixed:
+            for k in ("phase_minus", "phase_plus"):
+                self._kernel_tables[k].copy_(D[k].to(torch.complex64))
+
     # ---------------------------------------------------------------- structs
     def _fb(self, name):
         st = self._fcache.get(name)

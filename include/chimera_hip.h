@@ -389,6 +389,37 @@ int chm_laser_spectral_f64(const chm_spectral_t *sp, const double *w_m0,
                            void *Ez_fb_m0, void *Gz_fb_m0, double X_focus,
                            void *stream);
 
+/* ------------------------------------------------------------ moving window */
+/* ParticleMethodsCL.make_new_domain (methods/particles_methods_cl.py:66-147)
+ * with fill_grid (kernels/particles_generic.cl:33-84): for every cell of the
+ * local grids xgrid[Nx_loc] x rgrid (ncells = (Nx_loc-1)(Nr_loc-1)),
+ * npt x npr x npx particles at the regular sub-positions, the theta set
+ * rotated by a per-cell uniform offset in [0, 2 pi); w = r * w0; momenta
+ * (px_c, py_c, pz_c) and g_inv = 1/sqrt(1 + p^2) when px is not NULL.
+ * The offset of cell c is 2 pi u with u the top 53 bits of splitmix64 at
+ * (seed, counter0 + c) (the reference draws it from PyOpenCL's generator,
+ * which is not reproducible here); sin/cos are evaluated from IEEE +, -, *,
+ * / only, so a CPU restatement reproduces every bit. */
+int chm_fill_domain_f64(double *x, double *y, double *z, double *w, double *px,
+                        double *py, double *pz, double *g_inv,
+                        const double *xgrid, const double *rgrid,
+                        int32_t Nx_loc, int64_t ncells, int32_t npx,
+                        int32_t npr, int32_t npt, double w0, double px_c,
+                        double py_c, double pz_c, uint64_t seed,
+                        uint64_t counter0, void *stream);
+int chm_fill_domain_f32(float *x, float *y, float *z, float *w, float *px,
+                        float *py, float *pz, float *g_inv,
+                        const double *xgrid, const double *rgrid,
+                        int32_t Nx_loc, int64_t ncells, int32_t npx,
+                        int32_t npr, int32_t npt, double w0, double px_c,
+                        double py_c, double pz_c, uint64_t seed,
+                        uint64_t counter0, void *stream);
+/* get_phase_minus / get_phase_plus (kernels/transformer_generic.cl:28-55)
+ * for the window origin xmin after Frame.shift_grids (frame.py:22-30): the
+ * transforms read these tables instead of recomputing them per call. */
+int chm_phase_tables_f64(const double *kx, int32_t Nx, double xmin,
+                         void *phase_minus, void *phase_plus, void *stream);
+
 /* ------------------------------------------------------------------ adjoints
  * Reverse-mode kernels of the differentiable form (no counterpart in the
  * reference, whose step is not differentiable).  Adjoints of real values

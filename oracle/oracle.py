@@ -125,8 +125,13 @@ def particle_args(cfg):
     a["dt_2"] = 0.5 * a["dt"]
     if "Nppc" in a:
         nppc = np.array(a["Nppc"], dtype=np.uint32)
+        a["Nppc"] = nppc
         a["w0"] = 2 * np.pi * a["dx"] * a["dr"] * a["dens"] / np.prod(nppc)
+        a["ddx"] = a["dx"] / nppc[0]
+    else:
+        a["ddx"] = 1.0
     a["FactorPush"] = 2 * np.pi * a["dt"] * a["charge"] / a["mass"]
+    a["right_lim"] = 0.0
     return a
 
 
@@ -187,6 +192,46 @@ def get_m1(f):
 
 
 # --------------------------------------------------------------------------
+# deterministic generator and sin/cos of the domain fill
+# --------------------------------------------------------------------------
+def rng_u01(seed, counters):
+    """splitmix64 (Steele, Lea, Flood 2014) at seed + (counter+1)*golden, top
+    53 bits as a double in [0, 1); the GPU's chm_fill_domain draws the same."""
+    c = np.asarray(counters, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (c + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def det_sincos(a):
+    """sin, cos from IEEE +, -, *, / only: Cody-Waite reduction by pi/2 with
+    fdlibm's three-part constant, Taylor polynomials on |r| <= pi/4; the same
+    expressions in the same order as the GPU kernel, hence the same bits."""
+    a = np.asarray(a, dtype=np.float64)
+    P1, P2, P2T = 1.57079632673412561417e+00, 6.07710050630396597660e-11, 2.02226624879595063154e-21
+    k = np.floor(a * (2.0 / 3.14159265358979323846) + 0.5)
+    r = ((a - k * P1) - k * P2) - k * P2T
+    r2 = r * r
+    s = np.full_like(r, 1.0 / 355687428096000.0)
+    for cf in (-1.0 / 1307674368000.0, 1.0 / 6227020800.0, -1.0 / 39916800.0, 1.0 / 362880.0,
+               -1.0 / 5040.0, 1.0 / 120.0, -1.0 / 6.0):
+        s = cf + r2 * s
+    s = r + r * (r2 * s)
+    c = np.full_like(r, -1.0 / 6402373705728000.0)
+    for cf in (1.0 / 20922789888000.0, -1.0 / 87178291200.0, 1.0 / 479001600.0, -1.0 / 3628800.0,
+               1.0 / 40320.0, -1.0 / 720.0, 1.0 / 24.0, -0.5):
+        c = cf + r2 * c
+    c = 1.0 + r2 * c
+    q = k.astype(np.int64) & 3
+    sv = np.select([q == 0, q == 1, q == 2], [s, c, -s], -c)
+    cv = np.select([q == 0, q == 1, q == 2], [c, -s, -c], s)
+    return sv, cv
+
+
+# --------------------------------------------------------------------------
 # particles
 # --------------------------------------------------------------------------
 class OracleParticles:
@@ -212,15 +257,90 @@ class OracleParticles:
         self.Args["Np_stay"] = self.Args["Np"]
         self.flag_sorted = False
 
-    def dens_profile(self, x_prf, f_prf):
+    def dens_profile(self, x_prf, f_prf, xmin=None, xmax=None, coord="x", weight="w"):
         """dens_profile + profile_by_interpolant, particles_methods_cl.py:179-204,
         particles_generic.cl:6-30 (constant extension outside the knots)."""
         xx = np.ascontiguousarray(x_prf, dtype=np.float64)
         ff = np.ascontiguousarray(f_prf, dtype=np.float64)
         dxm1 = np.ascontiguousarray(1.0 / (xx[1:] - xx[:-1]))
         d = self.data
-        lib().orc_profile_by_interpolant(_p(d["x"]), _p(d["w"]), d["x"].size,
+        lib().orc_profile_by_interpolant(_p(d[coord]), _p(d[weight]), d[coord].size,
                                          _p(xx), _p(ff), _p(dxm1), xx.size)
+
+    def _keys(self):
+        return ["x", "y", "z", "w"] if self.immobile else \
+            ["x", "y", "z", "px", "py", "pz", "w", "g_inv"]
+
+    def make_new_domain(self, parts_in, density_profiles=None):
+        """make_new_domain, particles_methods_cl.py:66-147, with fill_grid,
+        particles_generic.cl:33-84 (theta offsets from rng_u01, sin/cos from
+        det_sincos, as the GPU fills it)."""
+        a = self.Args
+        xmin, xmax, rmin, rmax = (parts_in[k] for k in ("Xmin", "Xmax", "Rmin", "Rmax"))
+        Nx_loc = int(np.ceil((xmax - xmin) / a["dx"]) + 1)
+        Nr_loc = int(np.round((rmax - rmin) / a["dr"]) + 1)
+        xg = xmin + a["dx"] * np.arange(Nx_loc)
+        rg = rmin + a["dr"] * np.arange(Nr_loc)
+        a["right_lim"] = float(xg[-1])
+        ncells = (Nx_loc - 1) * (Nr_loc - 1)
+        npx, npr, npt = (int(v) for v in a["Nppc"])
+        counter = int(a.get("rng_counter", 0))
+        cell = np.arange(ncells, dtype=np.int64)
+        ir, ix = cell // (Nx_loc - 1), cell % (Nx_loc - 1)
+        thmin = (2.0 * 3.14159265358979323846) * rng_u01(int(a.get("seed", 0)), counter + cell)
+        a["rng_counter"] = counter + ncells
+        Lx = xg[ix + 1] - xg[ix]
+        Lr = rg[ir + 1] - rg[ir]
+        dx, dr, dth = 1. / float(npx), 1. / float(npr), 2 * 3.14159265358979323846 / float(npt)
+        it = np.arange(npt)[None, :, None, None]
+        irr = np.arange(npr)[None, None, :, None]
+        ixx = np.arange(npx)[None, None, None, :]
+        th = thmin[:, None, None, None] + it * dth
+        sin_th, cos_th = det_sincos(th)
+        rp = rg[ir][:, None, None, None] + (0.5 + irr) * dr * Lr[:, None, None, None]
+        shp = (ncells, npt, npr, npx)
+        x = np.broadcast_to(xg[ix][:, None, None, None] + (0.5 + ixx) * dx * Lx[:, None, None, None], shp)
+        new = {"x": x.ravel().copy(), "y": np.broadcast_to(rp * sin_th, shp).ravel().copy(),
+               "z": np.broadcast_to(rp * cos_th, shp).ravel().copy(),
+               "w": np.broadcast_to(rp * a["w0"], shp).ravel().copy()}
+        if not self.immobile:
+            pc = [float(parts_in.get(k + "_c", 0.0)) for k in ("px", "py", "pz")]
+            n = new["x"].size
+            for k, v in zip(("px", "py", "pz"), pc):
+                new[k] = np.full(n, v)
+            g = 1.0
+            if any(v != 0.0 for v in pc):
+                g = 1. / np.sqrt(1 + pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2])
+            new["g_inv"] = np.full(n, g)
+        for k, v in new.items():
+            self.data[k + "_new"] = v
+        if density_profiles is not None:
+            for profile in density_profiles:
+                if profile["coord"] == "x":
+                    self.dens_profile(profile["points"], profile["values"], coord="x_new",
+                                      weight="w_new")
+
+    def add_new_particles(self, source=None):
+        """add_new_particles, particles_methods_cl.py:35-61."""
+        src = (self if source is None else source).data
+        self.add_particles({k: src[k + "_new"] for k in self._keys()})
+
+    def free_added(self):
+        for k in list(self.data):
+            if k.endswith("_new"):
+                del self.data[k]
+
+    def align_parts(self):
+        """align_parts -> align_and_damp, particles.py:42-51,
+        particles_methods_cl.py:263-293: keep the Np_stay particles of the
+        valid cells, in sorted order."""
+        if self.Args["Np"] == 0:
+            return
+        idx = self.data["sort_indx"][: self.Args["Np_stay"]].astype(np.int64)
+        for k in self._keys():
+            self.data[k] = np.ascontiguousarray(self.data[k][idx])
+        self.Args["Np"] = self.Args["Np_stay"] = idx.size
+        self.flag_sorted = False
 
     def push_coords(self, mode="half"):
         """push_coords, particles_methods_cl.py:206-223."""
@@ -559,16 +679,55 @@ def add_gausian_pulse(solver, laser):
     solver.fb_transform(vects=["B", "E"], dir=1)
 
 
+class OracleFrame:
+    """Frame, frame.py:4-64 (moving window and plasma injection)."""
+
+    def __init__(self, cfg):
+        self.Args = dict(cfg)
+        self.Args.setdefault("Steps", 1.)
+        self.Args.setdefault("Velocity", 0.)
+        self.Args.setdefault("dt", 1)
+        self.Args.setdefault("DensityProfiles", None)
+
+    def shift_grids(self, grids=(), steps=None):
+        steps = self.Args["Steps"] if steps is None else steps
+        x_shift = steps * self.Args["dt"] * self.Args["Velocity"]
+        for g in grids:
+            for k in ("Xmax", "Xmin", "Xgrid"):
+                g.Args[k] = g.Args[k] + x_shift
+
+    def inject_plasma(self, species, grid, steps=None):
+        steps = self.Args["Steps"] if steps is None else steps
+        x_shift = steps * self.Args["dt"] * self.Args["Velocity"]
+        for sp in species:
+            if sp.Args["Np"] == 0:
+                sp.Args["right_lim"] = grid.Args["Xmax"] - x_shift
+            dom = {"Xmin": sp.Args["right_lim"]}
+            dom["Xmax"] = dom["Xmin"] + x_shift
+            dom["Rmin"] = grid.Args["Rmin"] * (grid.Args["Rmin"] > 0)
+            dom["Rmax"] = grid.Args["Rmax"]
+            sp.make_new_domain(dom, density_profiles=self.Args["DensityProfiles"])
+            sp.add_new_particles(sp.Args.get("InjectorSource"))
+        for sp in species:
+            sp.free_added()
+            sp.sort_parts(grid)
+            sp.align_parts()
+            num_ppc = int(np.prod(sp.Args["Nppc"]) + 1)
+            sp.Args["right_lim"] = float(sp.data["x"][-num_ppc:].max()) + 0.5 * sp.Args["ddx"]
+
+
 class OracleLoop:
-    """PIC_loop.step, pic_loop.py:57-142 (no frames, no diagnostics).
+    """PIC_loop.step, pic_loop.py:57-142 (frames, no diagnostics)."""
 
-    ``record`` keeps the charge-conservation inputs of the last step."""
-
-    def __init__(self, solver, species):
-        self.solver, self.species, self.it = solver, species, 0
+    def __init__(self, solver, species, frames=()):
+        self.solver, self.species, self.frames, self.it = solver, species, list(frames), 0
 
     def step(self):
         s = self.solver
+        for fr in self.frames:
+            if np.mod(self.it, fr.Args["Steps"]) == 0:
+                fr.shift_grids(grids=[s])
+                fr.inject_plasma(self.species, s)
         for p in self.species:
             p.push_coords("half")
             p.sort_parts(s)

@@ -1,0 +1,89 @@
+"""Moving window (Frame, frame.py:4-64): the GPU domain fill reproduces the
+oracle's bit for bit; a laser-plasma run with window shifts and plasma
+injection matches the oracle's run."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_fill_domain_bit_exact(precision):
+    _require_gpu()
+    import chimeracl_amd as CA
+    from chimeracl_amd.methods.generic_methods_hip import Communicator
+    from oracle import oracle as O
+    cfg = {"Nppc": (2, 3, 4), "dx": 0.02, "dr": 0.1, "dt": 0.02, "dens": 0.002, "charge": -1,
+           "seed": 5}
+    comm = Communicator(precision=precision)
+    g = CA.Particles(dict(cfg), comm)
+    o = O.OracleParticles(dict(cfg))
+    dom = {"Xmin": -1.37, "Xmax": -1.21, "Rmin": 0.0, "Rmax": 2.25, "px_c": 0.3}
+    for _ in range(2):  # the counter advances between fills
+        g.make_new_domain(dict(dom))
+        o.make_new_domain(dict(dom))
+        for k in ("x", "y", "z", "w", "px", "py", "pz", "g_inv"):
+            a = g.DataDev[k + "_new"].cpu().numpy()
+            b = o.data[k + "_new"]
+            if precision == "mixed":
+                b = b.astype(np.float32)
+            assert a.shape == b.shape and np.array_equal(a, b), k
+    assert g.Args["rng_counter"] == o.Args["rng_counter"]
+
+
+def _frame_pair(precision="f64"):
+    import chimeracl_amd as CA
+    from chimeracl_amd import config as C
+    from chimeracl_amd.methods.generic_methods_hip import Communicator
+    from oracle import oracle as O
+    from tests import simpair as S
+    g = S.tiny_grid(Nx=64, Nr=24, M=1, damp=4)
+    las = S.tiny_laser(g)
+    comm = Communicator(precision=precision)
+    solver = CA.Solver(dict(g), comm)
+    CA.add_gausian_pulse(solver, dict(las))
+    eo, io = C.species_config("C1", solver.Args, ppc=(2, 2, 2), dens=0.01)
+    eo["Nppc"] = io["Nppc"] = (2, 2, 2)
+    eo["dens"] = io["dens"] = 0.01
+    arr = C.load_plasma(g, (2, 2, 2), 0.01, seed=0)
+    eons, ions = CA.Particles(dict(eo), comm), CA.Particles(dict(io), comm)
+    eons.add_particles(source=C.ParticleSource(arr))
+    ions.add_particles(source=C.ParticleSource(arr))
+    ions.Args["InjectorSource"] = eons
+    prof = [{"coord": "x", "points": [-10.0, 0.5, 1.0, 10.0], "values": [1.0, 1.0, 1.5, 1.5]}]
+    fr_cfg = {"Velocity": 1.0, "dt": g["dt"], "Steps": 3, "DensityProfiles": prof}
+    frame = CA.Frame(dict(fr_cfg))
+    loop = CA.PIC_loop(solvers=[solver], species=[eons, ions], frames=[frame])
+    s = O.OracleSolver(dict(g))
+    O.add_gausian_pulse(s, dict(las))
+    e, i = O.OracleParticles(dict(eo)), O.OracleParticles(dict(io))
+    e.add_particles(arr)
+    i.add_particles(arr)
+    i.Args["InjectorSource"] = e
+    oloop = O.OracleLoop(s, [e, i], frames=[O.OracleFrame(dict(fr_cfg))])
+    return (solver, eons, ions, loop), (s, e, i, oloop)
+
+
+def test_moving_window_run_matches_oracle():
+    _require_gpu()
+    from tests import simpair as S
+    (solver, eons, ions, loop), (s, e, i, oloop) = _frame_pair()
+    np0 = eons.Args["Np"]
+    for _ in range(9):
+        loop.step()
+        oloop.step()
+    assert solver.Args["Xmin"] == s.Args["Xmin"]
+    assert eons.Args["Np"] == e.Args["Np"] and ions.Args["Np"] == i.Args["Np"]
+    assert eons.Args["Np"] != np0  # particles were dropped and injected
+    assert eons.Args["right_lim"] == pytest.approx(e.Args["right_lim"], abs=1e-12)
+    for k in ("x", "y", "z", "px", "py", "pz", "w"):
+        assert S.rel_l2(eons.host(k), e.data[k]) < 1e-10, k
+    errs = S.compare_fields(solver, s, S.real_names(1))
+    assert max(errs.values()) < 1e-10, errs

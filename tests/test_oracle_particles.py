@@ -287,3 +287,56 @@ def test_c_deposit_equals_python_restatement(oracle_mod):
     m0, m1 = _py_deposit_scalar(s, arr, p.data["sort_indx"], p.data["cell_offset"], -1.0)
     assert np.array_equal(s.f["rho_m0"], m0)
     assert np.array_equal(s.f["rho_m1"], m1)
+
+
+def test_domain_fill_generator_known_answers(oracle_mod):
+    """splitmix64's published first outputs for seed 0 (Steele, Lea, Flood
+    2014; the reference implementation's test vector)."""
+    O = oracle_mod
+    words = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    u = O.rng_u01(0, np.arange(3))
+    assert np.array_equal(u, np.array([(w >> 11) / 2.0 ** 53 for w in words]))
+
+
+def test_det_sincos_matches_libm(oracle_mod):
+    """The IEEE-only sin/cos of the domain fill agrees with numpy's to 2 ulp
+    on the angles it is used for ([0, 4 pi)) and beyond."""
+    O = oracle_mod
+    a = np.concatenate([np.linspace(0, 4 * np.pi, 200001), np.random.default_rng(3).uniform(-50, 50, 100000),
+                        np.array([0.0, np.pi / 4, np.pi / 2, np.pi, 1.5 * np.pi, 2 * np.pi])])
+    s, c = O.det_sincos(a)
+    es, ec = np.sin(a), np.cos(a)
+    tol = lambda ref: 2 * np.spacing(np.maximum(np.abs(ref), 1e-16)) + 2e-16
+    assert np.all(np.abs(s - es) <= tol(es))
+    assert np.all(np.abs(c - ec) <= tol(ec))
+    assert np.all(s * s + c * c - 1 < 4e-16)
+
+
+def test_make_new_domain_geometry(oracle_mod):
+    """fill_grid: regular sub-positions in every local cell, one theta offset
+    per cell, w = w0 r; counters advance so two fills differ."""
+    O = oracle_mod
+    p = O.OracleParticles({"Nppc": (2, 2, 4), "dx": 0.02, "dr": 0.1, "dt": 0.02, "dens": 0.002,
+                           "charge": -1})
+    dom = {"Xmin": -0.4, "Xmax": -0.2, "Rmin": 0.0, "Rmax": 1.05}
+    p.make_new_domain(dom)
+    d = p.data
+    Nx_loc = int(np.ceil((-0.2 + 0.4) / 0.02) + 1)
+    Nr_loc = int(np.round(1.05 / 0.1) + 1)
+    n = (Nx_loc - 1) * (Nr_loc - 1) * 16
+    assert d["x_new"].size == n
+    x, r = d["x_new"], np.hypot(d["y_new"], d["z_new"])
+    cell = np.arange(n) // 16
+    ix, ir = cell % (Nx_loc - 1), cell // (Nx_loc - 1)
+    sub = np.arange(n) % 2
+    assert np.allclose(x, -0.4 + 0.02 * ix + (0.5 + sub) * 0.01, atol=1e-13)
+    sub_r = (np.arange(n) // 2) % 2
+    assert np.allclose(r, 0.1 * ir + (0.5 + sub_r) * 0.05, atol=1e-13)
+    assert np.allclose(d["w_new"], p.Args["w0"] * r, rtol=1e-13)
+    th = np.arctan2(d["y_new"], d["z_new"]).reshape(-1, 4, 2, 2)[:, :, 0, 0]
+    dth = np.mod(np.diff(th, axis=1), 2 * np.pi)
+    assert np.allclose(dth, np.pi / 2, atol=1e-12)
+    assert np.all(d["g_inv_new"] == 1) and np.all(d["px_new"] == 0)
+    first = d["y_new"].copy()
+    p.make_new_domain(dom)
+    assert not np.array_equal(first, p.data["y_new"])
