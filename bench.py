@@ -322,8 +322,7 @@ def moving_window(sim, events=4, steps_per_event=20):
     s = sim.solver
     frame = CA.Frame({"Velocity": 1.0, "dt": s.Args["dt"], "Steps": steps_per_event})
     sim.ions.Args["InjectorSource"] = sim.eons
-    loop = CA.PIC_loop(solvers=[s], species=[sim.eons, sim.ions], frames=[frame])
-    loop.it = 1
+    loop = CA.PIC_loop(solvers=[s], species=[sim.eons, sim.ions])  # events driven here
     t_ev = t_all = 0.0
     n0 = sim.eons.Args["Np"]
     for e in range(events):
