@@ -51,6 +51,10 @@ class TablesOut(ctypes.Structure):
                 ("DHT", _MP), ("DHT_inv", _MP), ("dDp", _MP), ("dDm", _MP)]
 
 
+class Beam(ctypes.Structure):
+    _fields_ = [("c", f64 * 3), ("L", f64 * 3), ("pc", f64 * 3), ("dp", f64 * 3), ("w", f64)]
+
+
 R = ctypes.POINTER
 _SIGS = {
     "chm_push_x_f64": [P, P, P, P, P, P, P, f64, i64, P],
@@ -105,6 +109,8 @@ _SIGS = {
     "chm_fill_domain_f32": [P, P, P, P, P, P, P, P, P, P, i32, i64, i32, i32, i32, f64, f64, f64,
                             f64, ctypes.c_uint64, ctypes.c_uint64, P],
     "chm_phase_tables_f64": [P, i32, f64, P, P, P],
+    "chm_fill_beam_f64": [P, P, P, P, P, P, P, P, i64, R(Beam), ctypes.c_uint64, ctypes.c_uint64, P],
+    "chm_fill_beam_f32": [P, P, P, P, P, P, P, P, i64, R(Beam), ctypes.c_uint64, ctypes.c_uint64, P],
     # adjoints (differentiable form)
     "chm_push_x_adj_f64": [P, P, P, P, P, P, P, P, P, P, P, f64, i64, P],
     "chm_push_x_adj_f32": [P, P, P, P, P, P, P, P, P, P, P, f32, i64, P],

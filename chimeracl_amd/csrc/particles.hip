@@ -729,3 +729,93 @@ extern "C" int chm_fill_domain_f32(float *x, float *y, float *z, float *w, float
   return fill_domain<float>(x, y, z, w, px, py, pz, g_inv, xgrid, rgrid, Nx_loc, ncells, npx, npr,
                             npt, w0, pxc, pyc, pzc, seed, counter0, stream);
 }
+
+// ------------------------------------------------------------------ beam
+// log from IEEE operations and the exact frexp split, as the CPU oracle
+// evaluates it: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(f),
+// f = (m-1)/(m+1), series to f^19 (|f| <= 0.172); e ln2 with fdlibm's
+// two-part ln2.
+__device__ __forceinline__ double det_log(double x) {
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e -= 1;
+  }
+  const double f = (m - 1.0) / (m + 1.0);
+  const double f2 = f * f;
+  double s = 1.0 / 19.0;
+  s = (1.0 / 17.0) + f2 * s;
+  s = (1.0 / 15.0) + f2 * s;
+  s = (1.0 / 13.0) + f2 * s;
+  s = (1.0 / 11.0) + f2 * s;
+  s = (1.0 / 9.0) + f2 * s;
+  s = (1.0 / 7.0) + f2 * s;
+  s = (1.0 / 5.0) + f2 * s;
+  s = (1.0 / 3.0) + f2 * s;
+  const double lm = 2.0 * (f + f * (f2 * s));
+  const double de = (double)e;
+  return de * 6.93147180369123816490e-01 + (lm + de * 1.90821492927058770002e-10);
+}
+
+// Two independent standard normals from counters (c, c+1) (Box-Muller).
+__device__ __forceinline__ void det_normal2(uint64_t seed, uint64_t c, double *z0, double *z1) {
+  const double u1 = rng_u01(seed, c), u2 = rng_u01(seed, c + 1);
+  const double rad = sqrt(-2.0 * det_log(1.0 - u1));
+  double s, co;
+  det_sincos((2.0 * 3.14159265358979323846) * u2, &s, &co);
+  *z0 = rad * co;
+  *z1 = rad * s;
+}
+
+// make_new_beam, particles_methods_cl.py:149-177: Gaussian positions and
+// momenta, equal weights, g_inv from the momenta.  Particle p draws its six
+// normals from counters counter0 + 6p .. + 5.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_fill_beam(T *__restrict__ x, T *__restrict__ y, T *__restrict__ z, T *__restrict__ px,
+                T *__restrict__ py, T *__restrict__ pz, T *__restrict__ w, T *__restrict__ g_inv,
+                int64_t Np, chm_beam_t b, uint64_t seed, uint64_t counter0) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= Np) return;
+  const uint64_t c = counter0 + 6ull * (uint64_t)p;
+  double n[6];
+  det_normal2(seed, c, &n[0], &n[1]);
+  det_normal2(seed, c + 2, &n[2], &n[3]);
+  det_normal2(seed, c + 4, &n[4], &n[5]);
+  const double xv = b.c[0] + b.L[0] * n[0], yv = b.c[1] + b.L[1] * n[1],
+               zv = b.c[2] + b.L[2] * n[2];
+  const double ux = b.pc[0] + b.dp[0] * n[3], uy = b.pc[1] + b.dp[1] * n[4],
+               uz = b.pc[2] + b.dp[2] * n[5];
+  x[p] = (T)xv;
+  y[p] = (T)yv;
+  z[p] = (T)zv;
+  px[p] = (T)ux;
+  py[p] = (T)uy;
+  pz[p] = (T)uz;
+  w[p] = (T)b.w;
+  g_inv[p] = (T)(1. / sqrt(1 + ux * ux + uy * uy + uz * uz));
+}
+
+template <typename T>
+static int fill_beam(T *x, T *y, T *z, T *px, T *py, T *pz, T *w, T *g_inv, int64_t Np,
+                     const chm_beam_t *b, uint64_t seed, uint64_t counter0, void *stream) {
+  if (!b || Np < 0 || !x || !y || !z || !px || !py || !pz || !w || !g_inv) return CHM_EINVAL;
+  if (Np == 0) return CHM_OK;
+  k_fill_beam<T><<<chm_blocks(Np, 256), 256, 0, (hipStream_t)stream>>>(x, y, z, px, py, pz, w,
+                                                                       g_inv, Np, *b, seed, counter0);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_fill_beam_f64(double *x, double *y, double *z, double *px, double *py,
+                                 double *pz, double *w, double *g_inv, int64_t Np,
+                                 const chm_beam_t *b, uint64_t seed, uint64_t counter0,
+                                 void *stream) {
+  return fill_beam<double>(x, y, z, px, py, pz, w, g_inv, Np, b, seed, counter0, stream);
+}
+extern "C" int chm_fill_beam_f32(float *x, float *y, float *z, float *px, float *py, float *pz,
+                                 float *w, float *g_inv, int64_t Np, const chm_beam_t *b,
+                                 uint64_t seed, uint64_t counter0, void *stream) {
+  return fill_beam<float>(x, y, z, px, py, pz, w, g_inv, Np, b, seed, counter0, stream);
+}

@@ -98,10 +98,28 @@ class ParticleMethodsHIP(GenericMethodsHIP):
                                   weight="w_new")
 
     def make_new_beam(self, parts_in):
-        raise NotImplementedError(
-            "device-side random beam loading is not part of this build; pass arrays through "
-            "add_particles(source=...)")
-
+        """particles_methods_cl.py:149-177: Np Gaussian particles (x_c, Lx, ...,
+        px_c, dpx, ...; weights FullCharge/Np) into DataDev['<c>_new'], filled
+        on the GPU from the counter-based generator (Args 'seed',
+        advancing Args['rng_counter'] by 6 Np)."""
+        a = self.Args
+        Np = int(parts_in["Np"])
+        b = _lib.Beam()
+        for i, c in enumerate("xyz"):
+            b.c[i] = float(parts_in[c + "_c"])
+            b.L[i] = float(parts_in["L" + c])
+            b.pc[i] = float(parts_in.get("p" + c + "_c", 0.0))
+            b.dp[i] = float(parts_in.get("dp" + c, 0.0))
+        b.w = float(parts_in["FullCharge"]) / Np
+        dev, real = self.comm.device, self.comm.real
+        new = {k: torch.empty(Np, dtype=real, device=dev)
+               for k in ("x", "y", "z", "px", "py", "pz", "w", "g_inv")}
+        counter = int(a.get("rng_counter", 0))
+        self.call("chm_fill_beam", *(ptr(new[k]) for k in ("x", "y", "z", "px", "py", "pz", "w", "g_inv")),
+                  Np, b, int(a.get("seed", 0)), counter)
+        a["rng_counter"] = counter + 6 * Np
+        for k, t in new.items():
+            self.DataDev[k + "_new"] = t
     def reset_num_parts(self, Np=None):
         if Np is None:
             Np = int(self.DataDev["x"].numel())

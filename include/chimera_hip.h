@@ -414,6 +414,27 @@ int chm_fill_domain_f32(float *x, float *y, float *z, float *w, float *px,
                         int32_t npr, int32_t npt, double w0, double px_c,
                         double py_c, double pz_c, uint64_t seed,
                         uint64_t counter0, void *stream);
+/* ParticleMethodsCL.make_new_beam (methods/particles_methods_cl.py:149-177):
+ * Np particles with positions c + L n, momenta pc + dp n (n standard
+ * normals), weights w, g_inv = 1/sqrt(1 + p^2).  Particle p's six normals
+ * come from Box-Muller on splitmix64 at (seed, counter0 + 6p .. 6p + 5),
+ * with log, sin and cos evaluated from IEEE operations only (bit-exact
+ * against a CPU restatement). */
+typedef struct chm_beam {
+  double c[3];   /* x_c, y_c, z_c */
+  double L[3];   /* Lx, Ly, Lz */
+  double pc[3];  /* px_c, py_c, pz_c */
+  double dp[3];  /* dpx, dpy, dpz */
+  double w;      /* FullCharge / Np */
+} chm_beam_t;
+int chm_fill_beam_f64(double *x, double *y, double *z, double *px, double *py,
+                      double *pz, double *w, double *g_inv, int64_t Np,
+                      const chm_beam_t *b, uint64_t seed, uint64_t counter0,
+                      void *stream);
+int chm_fill_beam_f32(float *x, float *y, float *z, float *px, float *py,
+                      float *pz, float *w, float *g_inv, int64_t Np,
+                      const chm_beam_t *b, uint64_t seed, uint64_t counter0,
+                      void *stream);
 /* get_phase_minus / get_phase_plus (kernels/transformer_generic.cl:28-55)
  * for the window origin xmin after Frame.shift_grids (frame.py:22-30): the
  * transforms read these tables instead of recomputing them per call. */

@@ -231,6 +231,33 @@ def det_sincos(a):
     return sv, cv
 
 
+def det_log(x):
+    """log from IEEE operations and the exact frexp split (the GPU's det_log):
+    x = m 2^e, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh((m-1)/(m+1)) to f^19,
+    e ln2 with fdlibm's two-part ln2."""
+    m, e = np.frexp(np.asarray(x, dtype=np.float64))
+    low = m < 0.70710678118654752440
+    m = np.where(low, m * 2.0, m)
+    e = np.where(low, e - 1, e)
+    f = (m - 1.0) / (m + 1.0)
+    f2 = f * f
+    s = np.full_like(f, 1.0 / 19.0)
+    for cf in (1.0 / 17.0, 1.0 / 15.0, 1.0 / 13.0, 1.0 / 11.0, 1.0 / 9.0, 1.0 / 7.0, 1.0 / 5.0,
+               1.0 / 3.0):
+        s = cf + f2 * s
+    lm = 2.0 * (f + f * (f2 * s))
+    de = e.astype(np.float64)
+    return de * 6.93147180369123816490e-01 + (lm + de * 1.90821492927058770002e-10)
+
+
+def det_normal2(seed, c):
+    """Box-Muller pair from counters (c, c+1), as the GPU draws it."""
+    u1, u2 = rng_u01(seed, c), rng_u01(seed, c + 1)
+    rad = np.sqrt(-2.0 * det_log(1.0 - u1))
+    s, co = det_sincos((2.0 * 3.14159265358979323846) * u2)
+    return rad * co, rad * s
+
+
 # --------------------------------------------------------------------------
 # particles
 # --------------------------------------------------------------------------
@@ -319,6 +346,27 @@ class OracleParticles:
                 if profile["coord"] == "x":
                     self.dens_profile(profile["points"], profile["values"], coord="x_new",
                                       weight="w_new")
+
+    def make_new_beam(self, parts_in):
+        """make_new_beam, particles_methods_cl.py:149-177 (normals of particle p
+        from counters counter0 + 6p .. 6p+5, as the GPU fills them)."""
+        a = self.Args
+        Np = int(parts_in["Np"])
+        counter = int(a.get("rng_counter", 0))
+        c = counter + 6 * np.arange(Np, dtype=np.uint64)
+        n0, n1 = det_normal2(int(a.get("seed", 0)), c)
+        n2, n3 = det_normal2(int(a.get("seed", 0)), c + np.uint64(2))
+        n4, n5 = det_normal2(int(a.get("seed", 0)), c + np.uint64(4))
+        a["rng_counter"] = counter + 6 * Np
+        g = lambda k, d=0.0: float(parts_in.get(k, d))
+        new = {"x": g("x_c") + g("Lx") * n0, "y": g("y_c") + g("Ly") * n1, "z": g("z_c") + g("Lz") * n2,
+               "px": g("px_c") + g("dpx") * n3, "py": g("py_c") + g("dpy") * n4,
+               "pz": g("pz_c") + g("dpz") * n5}
+        new["w"] = np.full(Np, float(parts_in["FullCharge"]) / Np)
+        ux, uy, uz = new["px"], new["py"], new["pz"]
+        new["g_inv"] = 1. / np.sqrt(1 + ux * ux + uy * uy + uz * uz)
+        for k, v in new.items():
+            self.data[k + "_new"] = v
 
     def add_new_particles(self, source=None):
         """add_new_particles, particles_methods_cl.py:35-61."""

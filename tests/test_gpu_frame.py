@@ -87,3 +87,28 @@ def test_moving_window_run_matches_oracle():
         assert S.rel_l2(eons.host(k), e.data[k]) < 1e-10, k
     errs = S.compare_fields(solver, s, S.real_names(1))
     assert max(errs.values()) < 1e-10, errs
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_fill_beam_bit_exact(precision):
+    _require_gpu()
+    import chimeracl_amd as CA
+    from chimeracl_amd.methods.generic_methods_hip import Communicator
+    from oracle import oracle as O
+    cfg = {"dx": 0.02, "dr": 0.1, "dt": 0.02, "charge": -1, "seed": 11}
+    comm = Communicator(precision=precision)
+    g = CA.Particles(dict(cfg), comm)
+    o = O.OracleParticles(dict(cfg))
+    beam = {"Np": 50000, "x_c": -1.0, "Lx": 0.2, "y_c": 0.0, "Ly": 0.5, "z_c": 0.1, "Lz": 0.5,
+            "px_c": 100.0, "dpx": 5.0, "dpz": 1.0, "FullCharge": 3.0}
+    for _ in range(2):
+        g.make_new_beam(dict(beam))
+        o.make_new_beam(dict(beam))
+        for k in ("x", "y", "z", "px", "py", "pz", "w", "g_inv"):
+            a = g.DataDev[k + "_new"].cpu().numpy()
+            b = o.data[k + "_new"]
+            if precision == "mixed":
+                b = b.astype(np.float32)
+            assert np.array_equal(a, b), k
+    g.add_new_particles()
+    assert g.Args["Np"] == 50000

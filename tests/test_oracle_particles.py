@@ -340,3 +340,35 @@ def test_make_new_domain_geometry(oracle_mod):
     first = d["y_new"].copy()
     p.make_new_domain(dom)
     assert not np.array_equal(first, p.data["y_new"])
+
+
+def test_det_log_matches_libm(oracle_mod):
+    O = oracle_mod
+    x = np.concatenate([np.random.default_rng(4).random(200000), np.logspace(-300, 300, 20001),
+                        np.array([1.0, 0.5, 2.0, np.e, 1e-16, 1 - 2 ** -53])])
+    x = x[x > 0]
+    v, ref = O.det_log(x), np.log(x)
+    assert np.all(np.abs(v - ref) <= 2 * np.spacing(np.abs(ref)) + 1e-300)
+    assert O.det_log(np.array([1.0]))[0] == 0.0
+
+
+def test_beam_statistics(oracle_mod):
+    """make_new_beam: means and spreads as requested (Box-Muller normals)."""
+    O = oracle_mod
+    p = O.OracleParticles({"dx": 0.02, "dr": 0.1, "dt": 0.02, "charge": -1, "seed": 9})
+    Np = 400000
+    beam = {"Np": Np, "x_c": -1.0, "Lx": 0.2, "y_c": 0.0, "Ly": 0.5, "z_c": 0.1, "Lz": 0.5,
+            "px_c": 100.0, "dpx": 5.0, "dpz": 1.0, "FullCharge": 3.0}
+    p.make_new_beam(beam)
+    d = p.data
+    se = 5 / np.sqrt(Np)
+    for k, mu, sig in (("x", -1.0, 0.2), ("y", 0.0, 0.5), ("z", 0.1, 0.5), ("px", 100.0, 5.0),
+                       ("py", 0.0, 0.0), ("pz", 0.0, 1.0)):
+        v = d[k + "_new"]
+        assert abs(v.mean() - mu) <= se * max(sig, 1e-12) + 1e-14, k
+        assert abs(v.std() - sig) <= se * sig + 1e-14, k
+    assert np.allclose(d["w_new"], 3.0 / Np)
+    u2 = d["px_new"] ** 2 + d["py_new"] ** 2 + d["pz_new"] ** 2
+    assert np.allclose(d["g_inv_new"], 1 / np.sqrt(1 + u2))
+    # two normals of a pair are uncorrelated
+    assert abs(np.corrcoef(d["x_new"], d["y_new"])[0, 1]) < 5 / np.sqrt(Np)
