@@ -24,6 +24,9 @@ def __getattr__(name):
     if name == "add_gausian_pulse":
         from .laser import add_gausian_pulse
         return add_gausian_pulse
+    if name == "Diagnostics":
+        from .diagnostics import Diagnostics
+        return Diagnostics
     if name == "Frame":
         from .frame import Frame
         return Frame

@@ -67,6 +67,8 @@ class GridMethodsHIP(GenericMethodsHIP):
         which adds the same sums in the same order."""
         if parts.Args["Np"] <= 0:
             return
+        if not parts.flag_sorted:  # deposits read the sorted order of this grid
+            parts.sort_parts(self)
         d = parts.DataDev
         planes = self._planes_arr([dest_fld])
         if "Immobile" in parts.Args:
@@ -92,6 +94,8 @@ class GridMethodsHIP(GenericMethodsHIP):
         """grid_methods_cl.py:71-95 (adds w*g_inv*charge*u into '<vec_fld><c>_m*')."""
         if parts.Args["Np"] <= 0:
             return
+        if not parts.flag_sorted:
+            parts.sort_parts(self)
         d, ws = parts.DataDev, self._dep_ws()
         comps = self.Args["vec_comps"]
         self.call("chm_deposit_vector", ptr(d["sort_indx"]), ptr(d["cell_offset"]), ptr(d["x"]),

@@ -1,6 +1,8 @@
 """Moving window (Frame, frame.py:4-64): the GPU domain fill reproduces the
 oracle's bit for bit; a laser-plasma run with window shifts and plasma
 injection matches the oracle's run."""
+import os
+
 import numpy as np
 import pytest
 
@@ -112,3 +114,41 @@ def test_fill_beam_bit_exact(precision):
             assert np.array_equal(a, b), k
     g.add_new_particles()
     assert g.Args["Np"] == 50000
+
+
+def test_diagnostics_records(tmp_path, monkeypatch):
+    """Diagnostics (diagnostics.py): one record per interval with the
+    reference's paths; fields stacked [m0, Re m1, Im m1]; weights in pC."""
+    _require_gpu()
+    import chimeracl_amd as CA
+    from chimeracl_amd import diagnostics as Dg
+    monkeypatch.chdir(tmp_path)
+    (solver, eons, ions, loop), _ = _frame_pair()
+    diag = CA.Diagnostics({"Interval": 2, "ScalarFields": ["rho"], "VectorFields": ["E"],
+                           "Species": {"Components": ["x", "px", "w"],
+                                       "Selections": [("x", -0.5, None)]}},
+                          solver, species=[eons], frame=loop.frames[0], path="diags")
+    loop.diags = [diag]
+    for _ in range(3):
+        loop.step()
+    ext = ".h5" if Dg.h5py is not None else ".npz"
+    files = sorted(os.listdir(tmp_path / "diags"))
+    assert files == ["000000000" + ext, "000000002" + ext]
+    if ext == ".npz":
+        rec = np.load(tmp_path / "diags" / files[-1])
+        get = lambda k: rec[k.lstrip("/")]
+    else:
+        import h5py
+        rec = h5py.File(tmp_path / "diags" / files[-1], "r")
+        get = lambda k: rec[k][()]
+    a = solver.Args
+    assert int(get("/data/info/iteration")) == 2
+    assert float(get("/data/info/FrameVelocity")) == 1.0
+    assert np.array_equal(get("/data/info/Xgrid"), a["Xgrid"])
+    ex = get("/data/fields/Ex")
+    assert ex.shape == (3, a["Nr"], a["Nx"]) and ex.dtype == np.float32
+    assert get("/data/fields/rho").shape == (3, a["Nr"], a["Nx"])
+    x = get("/data/species/species_0/x")
+    assert x.size > 0 and np.all(x > -0.5)
+    w = get("/data/species/species_0/w")
+    assert w.size == x.size and np.all(w > 0)
