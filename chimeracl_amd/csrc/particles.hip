@@ -4,17 +4,16 @@
 // Sort design (replaces index_and_sum_in_cell + PyOpenCL cumsum + the
 // atomic `sort` kernel, particles_generic.cl:88-126, 186-200):
 //   1. key kernel: cell key per particle (optionally fused with the
-//      coordinate push) and a run-aggregated atomic histogram (one atomic per
-//      run of equal keys inside a wave, no return value used)
-//   2. exclusive scan of the histogram -> cell_offset, Np_stay
-//   3. stable LSD radix sort of the keys with the particle index as value,
+//      coordinate push), no atomics
+//   2. stable LSD radix sort of the keys with the particle index as value,
 //      in one to three passes of 8..11-bit digits (two passes up to 22-bit
 //      keys, i.e. 4M cells): per 4096-key tile an LDS digit histogram, one
 //      scan over [digit][tile], then a scatter that ranks keys inside the
 //      tile in index order (ballot-matched digits per wave, tagged per-wave
-//      counts in LDS).
-// The result is exactly numpy.argsort(key, kind='stable'); no float or
-// returning global atomics, no order fix-up.
+//      counts in LDS); the last pass also writes the sorted keys
+//   3. cell_offset[c] = lower bound of c in the sorted keys, Np_stay
+// The result is exactly numpy.argsort(key, kind='stable') and the
+// reference's cell_offset; no global atomics, no order fix-up.
 #include "chm_common.h"
 
 #define SORT_BS 256
@@ -106,53 +105,33 @@ __device__ __forceinline__ void run_add(uint32_t *slot_base, bool act, uint32_t 
 }
 
 // With PUSH, the coordinate push of push_xyz (same rounding order) is applied
-// first and the key is taken from the pushed position.  Besides the keys and
-// the cell histogram it adds up, per RS_TILE tile of particles, the counts
-// of the sort's first radix digit (nb0 bits; integer atomics, aggregated
-// over runs of equal digits in a wave), so the sort's first pass needs no
-// separate counting sweep.
+// first and the key is taken from the pushed position.  Keys only: the cell
+// offsets come from the sorted keys, so this pass has no atomics and its cost
+// does not depend on the particles' order.
 template <typename T, bool PUSH>
 __global__ void __launch_bounds__(SORT_BS) k_cell_key(T *__restrict__ x, T *__restrict__ y,
                                                       T *__restrict__ z, const T *__restrict__ px,
                                                       const T *__restrict__ py, const T *__restrict__ pz,
                                                       const T *__restrict__ g_inv, T dt, int64_t Np,
                                                       int Nx, int Nr, T xmin, T dx_inv, T rmin, T dr_inv,
-                                                      uint32_t *__restrict__ key_out,
-                                                      uint32_t *__restrict__ hist, int nb0,
-                                                      uint32_t *__restrict__ counts, int64_t nblk) {
+                                                      uint32_t *__restrict__ key_out) {
   const int Nx_loc = Nx - 1, Nr_loc = Nr - 1;
-  const uint32_t ovf = (uint32_t)(Nr_loc * Nx_loc);
-  const int lane = threadIdx.x & 63;
   const int64_t ip = (int64_t)blockIdx.x * SORT_BS + threadIdx.x;
-  const bool act = ip < Np;
-  uint32_t key = ovf;
-  if (act) {
-    T xp = x[ip], yp = y[ip], zp = z[ip];
-    if (PUSH) {
-      T dt_g = dt * g_inv[ip];
-      T dx = px[ip] * dt_g;
-      T dy = py[ip] * dt_g;
-      T dz = pz[ip] * dt_g;
-      xp = xp + dx;
-      yp = yp + dy;
-      zp = zp + dz;
-      x[ip] = xp;
-      y[ip] = yp;
-      z[ip] = zp;
-    }
-    key = cell_key<T>(xp, yp, zp, Nx_loc, Nr_loc, xmin, dx_inv, rmin, dr_inv);
-    key_out[ip] = key;
+  if (ip >= Np) return;
+  T xp = x[ip], yp = y[ip], zp = z[ip];
+  if (PUSH) {
+    T dt_g = dt * g_inv[ip];
+    T dx = px[ip] * dt_g;
+    T dy = py[ip] * dt_g;
+    T dz = pz[ip] * dt_g;
+    xp = xp + dx;
+    yp = yp + dy;
+    zp = zp + dz;
+    x[ip] = xp;
+    y[ip] = yp;
+    z[ip] = zp;
   }
-  run_add(hist, act && key != ovf, key, lane);
-  // a wave lies inside one tile (RS_TILE is a multiple of 64)
-  const int64_t tile = ((int64_t)blockIdx.x * SORT_BS) / RS_TILE;
-  const uint32_t d = key & (((uint32_t)1 << nb0) - 1u);
-  run_add(counts + tile, act, d * (uint32_t)nblk, lane);
-}
-
-__global__ void k_zero_u32(uint32_t *__restrict__ a, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = 0;
+  key_out[ip] = cell_key<T>(xp, yp, zp, Nx_loc, Nr_loc, xmin, dx_inv, rmin, dr_inv);
 }
 
 // ------------------------------------------------------------------ scan
@@ -224,14 +203,21 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_final(uint32_t *__restrict__ i
 }
 
 // ------------------------------------------------------------------ offsets
-// cell_offset[c] = exclusive scan of the histogram; the out-of-domain bin is
-// not counted in the histogram, so its start is Np_stay and the end is Np.
-__global__ void k_offsets(const uint32_t *__restrict__ pre, uint32_t nbins, uint32_t Np,
-                          uint32_t *__restrict__ cell_offset, uint32_t *__restrict__ np_stay) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < (int64_t)nbins) cell_offset[c] = pre[c];
-  if (c == (int64_t)nbins) cell_offset[c] = Np;
-  if (c == 0) *np_stay = pre[nbins - 1];
+// cell_offset[c] = first sorted position whose key is >= c, c = 0..nbins
+// (lower bound by bisection over the sorted keys), and
+// Np_stay = cell_offset[nbins - 1], the start of the out-of-domain bin.
+__global__ void k_offsets_sorted(const uint32_t *__restrict__ skey, int64_t Np, int64_t nbins,
+                                 uint32_t *__restrict__ cell_offset, uint32_t *__restrict__ np_stay) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > nbins) return;
+  int64_t lo = 0, hi = Np;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)skey[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  cell_offset[c] = (uint32_t)lo;
+  if (c == nbins - 1) *np_stay = (uint32_t)lo;
 }
 
 // ------------------------------------------------------------------ radix sort
@@ -381,20 +367,16 @@ static int key_bits(int64_t nbins) {
 
 // ------------------------------------------------------------------ driver
 struct SortWs {
-  uint32_t *hist, *pre, *bsum, *counts, *offs, *bsum2, *kA, *kB, *vA, *vB;
-  int64_t nscan, nblk, ncounts, nscan2;
+  uint32_t *counts, *offs, *bsum2, *kA, *kB, *vA, *vB;
+  int64_t nblk, ncounts, nscan2;
 };
 
 static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
   int64_t nblk = (Np + RS_TILE - 1) / RS_TILE;
   if (nblk < 1) nblk = 1;
   int64_t ncounts = ((int64_t)1 << rs_plan(key_bits(nbins)).rb) * nblk;
-  int64_t nscan = (nbins + SCAN_TILE - 1) / SCAN_TILE;
   int64_t nscan2 = (ncounts + SCAN_TILE - 1) / SCAN_TILE;
   size_t off = 0;
-  size_t o_hist = off; off += chm_align(nbins * 4);
-  size_t o_pre = off; off += chm_align((nbins + 1) * 4);
-  size_t o_bsum = off; off += chm_align(nscan * 4);
   size_t o_counts = off; off += chm_align(ncounts * 4);
   size_t o_offs = off; off += chm_align((ncounts + 1) * 4);
   size_t o_bsum2 = off; off += chm_align(nscan2 * 4);
@@ -402,9 +384,6 @@ static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
   size_t o_v = off; off += 2 * chm_align(Np * 4);
   if (w) {
     char *b = (char *)base;
-    w->hist = (uint32_t *)(b + o_hist);
-    w->pre = (uint32_t *)(b + o_pre);
-    w->bsum = (uint32_t *)(b + o_bsum);
     w->counts = (uint32_t *)(b + o_counts);
     w->offs = (uint32_t *)(b + o_offs);
     w->bsum2 = (uint32_t *)(b + o_bsum2);
@@ -414,7 +393,6 @@ static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
     w->vB = (uint32_t *)(b + o_v + chm_align(Np * 4));
     w->nblk = nblk;
     w->ncounts = ncounts;
-    w->nscan = nscan;
     w->nscan2 = nscan2;
   }
   return off;
@@ -424,7 +402,7 @@ extern "C" size_t chm_sort_workspace_bytes(int64_t Np, int64_t nbins) {
   return sort_layout(Np, nbins, nullptr, nullptr);
 }
 
-// Stage 1: zero the histogram, then keys + histogram (+ the push with PUSH).
+// Stage 1: the cell keys (+ the push with PUSH).
 template <typename T, bool PUSH>
 static int sort_keys(T *x, T *y, T *z, const T *px, const T *py, const T *pz, const T *g_inv, T dt,
                      int64_t Np, const chm_grid_t *g, uint32_t *indx_in_cell, void *ws, size_t ws_bytes,
@@ -434,14 +412,10 @@ static int sort_keys(T *x, T *y, T *z, const T *px, const T *py, const T *pz, co
   SortWs w;
   if (sort_layout(Np, nbins, &w, ws) > ws_bytes || !ws) return CHM_EWORKSPACE;
   hipStream_t st = (hipStream_t)stream;
-  const int nb0 = rs_plan(key_bits(nbins)).nb[0];
-  const int64_t nc0 = ((int64_t)1 << nb0) * w.nblk;
-  k_zero_u32<<<chm_blocks(nbins, 256), 256, 0, st>>>(w.hist, nbins);
-  k_zero_u32<<<chm_blocks(nc0, 256), 256, 0, st>>>(w.counts, nc0);
   if (Np > 0)
     k_cell_key<T, PUSH><<<chm_blocks(Np, SORT_BS), SORT_BS, 0, st>>>(
         x, y, z, px, py, pz, g_inv, dt, Np, g->Nx, g->Nr, (T)g->xmin, (T)g->dx_inv, (T)g->rmin,
-        (T)g->dr_inv, indx_in_cell, w.hist, nb0, w.counts, w.nblk);
+        (T)g->dr_inv, indx_in_cell);
   CHM_CHECK_LAUNCH();
   return CHM_OK;
 }
@@ -452,7 +426,8 @@ static void scan_u32(const uint32_t *in, int64_t n, uint32_t *out, uint32_t *bsu
   k_scan_final<<<(unsigned)nscan, SCAN_BS, 0, st>>>((uint32_t *)in, n, 0, bsum, out);
 }
 
-// Stage 2: offsets and the stable radix sort, from the keys of stage 1.
+// Stage 2: the stable radix sort of the keys of stage 1, then the offsets
+// from the sorted keys.
 static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_in_cell,
                           uint32_t *cell_offset, uint32_t *sort_indx, uint32_t *np_stay, void *ws,
                           size_t ws_bytes, void *stream) {
@@ -461,41 +436,36 @@ static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_
   SortWs w;
   if (sort_layout(Np, nbins, &w, ws) > ws_bytes || !ws) return CHM_EWORKSPACE;
   hipStream_t st = (hipStream_t)stream;
-  scan_u32(w.hist, nbins, w.pre, w.bsum, w.nscan, st);
-  k_offsets<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(w.pre, (uint32_t)nbins, (uint32_t)Np,
-                                                        cell_offset, np_stay);
-  if (Np == 0) {
-    CHM_CHECK_LAUNCH();
-    return CHM_OK;
-  }
-  const RsPlan plan = rs_plan(key_bits(nbins));
-  const uint32_t *kin = indx_in_cell, *vin = nullptr;
-  int shift = 0;
-  for (int p = 0; p < plan.npass; p++) {
-    const int nb = plan.nb[p];
-    const bool last = p == plan.npass - 1;
-    uint32_t *kout = last ? nullptr : ((p & 1) ? w.kB : w.kA);
-    uint32_t *vout = last ? sort_indx : ((p & 1) ? w.vB : w.vA);
-    const int64_t nc = ((int64_t)1 << nb) * w.nblk;
-    const int64_t nsc = (nc + SCAN_TILE - 1) / SCAN_TILE;
-    // the first digit's tile counts come from the key kernel
-    if (p > 0) {
+  const uint32_t *skey = indx_in_cell;
+  if (Np > 0) {
+    const RsPlan plan = rs_plan(key_bits(nbins));
+    const uint32_t *kin = indx_in_cell, *vin = nullptr;
+    int shift = 0;
+    for (int p = 0; p < plan.npass; p++) {
+      const int nb = plan.nb[p];
+      const bool last = p == plan.npass - 1;
+      uint32_t *kout = (p & 1) ? w.kB : w.kA;
+      uint32_t *vout = last ? sort_indx : ((p & 1) ? w.vB : w.vA);
+      const int64_t nc = ((int64_t)1 << nb) * w.nblk;
+      const int64_t nsc = (nc + SCAN_TILE - 1) / SCAN_TILE;
       if (plan.rb == 8)
         k_rs_upsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
       else
         k_rs_upsweep<11><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
+      scan_u32(w.counts, nc, w.offs, w.bsum2, nsc, st);
+      if (plan.rb == 8)
+        k_rs_downsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs,
+                                                              w.nblk, kout, vout);
+      else
+        k_rs_downsweep<11><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs,
+                                                               w.nblk, kout, vout);
+      kin = kout;
+      vin = vout;
+      shift += nb;
     }
-    scan_u32(w.counts, nc, w.offs, w.bsum2, nsc, st);
-    if (plan.rb == 8)
-      k_rs_downsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs, w.nblk,
-                                                            kout, vout);
-    else
-      k_rs_downsweep<11><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs,
-                                                             w.nblk, kout, vout);
-    kin = kout;
-    vin = vout;
-    shift += nb;
+    skey = kin;
   }
+  k_offsets_sorted<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset, np_stay);
   CHM_CHECK_LAUNCH();
   return CHM_OK;
 }

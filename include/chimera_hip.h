@@ -159,7 +159,7 @@ int chm_push_x_f32(float *x, float *y, float *z, const float *px,
                    const float *py, const float *pz, const float *g_inv,
                    float dt, int64_t Np, void *stream);
 
-/* Cell keys, histogram, offsets and the stable sort of index_sort.
+/* Cell keys, the stable sort and the offsets of index_sort.
  * nbins = (Nx-1)*(Nr-1)+1 (last bin = particles outside the valid cells).
  * Outputs: indx_in_cell [Np] (cell key), cell_offset [nbins+1]
  * (cell_offset[c] = first sorted slot of bin c, cell_offset[nbins] = Np),
@@ -180,8 +180,7 @@ int chm_index_sort_f32(const float *x, const float *y, const float *z,
 /* The two halves of index_sort, for the step's push-then-sort sequence
  * (pic_loop.py:70-91, push_coords followed by sort_parts):
  * chm_push_x_keys_* applies the coordinate push of chm_push_x_* and writes
- * the cell keys, the cell histogram and the per-tile counts of the sort's
- * first radix digit into the sort workspace in the same pass;
+ * the cell keys in the same pass (ws is checked for size only);
  * chm_sort_from_keys then produces cell_offset, sort_indx and np_stay
  * exactly as chm_index_sort_* does. */
 int chm_push_x_keys_f64(double *x, double *y, double *z, const double *px,
