@@ -592,17 +592,19 @@ __global__ void __launch_bounds__(256)
 // The same gather + push visiting sorted slots ip < Np_stay with
 // p = sort_indx[ip] (the reference kernel's own iteration, grid_deposit_m1.cl:
 // 330-350, skipping p >= Np_stay).  A block of GS_NT slots covers a short run
-// of consecutive cells; when it spans at most MAXC cells, the E and B node
-// values of those cells (4 nodes x NV values each) are staged into LDS once
-// and every lane reads its 72 (M = 1) field values from there instead of from
-// 18 separate planes.  Identical operands and operation order: the results
-// equal k_gather_push bit for bit.
+// of consecutive cells; when those cells lie in one grid row and span at most
+// MAXN - 1 cells, the E and B values of the two node rows under them (NV
+// values per node, modes pre-doubled) are staged into LDS once, one node per
+// lane with all of its loads in flight together, and every lane reads its 72
+// (M = 1) field values from there instead of from 18 separate planes.
+// Identical operands and operation order: the results equal k_gather_push
+// bit for bit.
 #define GS_NT 256
 template <typename T, int M> struct GatherCfg {
   static constexpr int NQ = 1 + 2 * M;
   static constexpr int NV = 6 * NQ;  // E and B, 3 components, all modes
-  static constexpr int MC = 20480 / (4 * NV * (int)sizeof(T));
-  static constexpr int MAXC = MC > 4 ? MC : 4;
+  static constexpr int MN = 20480 / (2 * NV * (int)sizeof(T));
+  static constexpr int MAXN = MN > GS_NT / 2 ? GS_NT / 2 : (MN < 4 ? 4 : MN);
 };
 
 template <typename T, int M>
@@ -649,6 +651,29 @@ __device__ __forceinline__ void gather_sum(LD ld, const T (&C)[2][2], const T *g
       }
 }
 
+// All NV values of one node into dst (GatherCfg order, modes doubled): the
+// loads are issued together, then written to LDS.
+template <typename T, int M>
+__device__ __forceinline__ void stage_node(const FieldPtrs &fp, int64_t node, T *dst) {
+  constexpr int NQ = GatherCfg<T, M>::NQ, NV = GatherCfg<T, M>::NV;
+  T val[NV];
+#pragma unroll
+  for (int f = 0; f < 2; f++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int vb = (3 * f + k) * NQ;
+      val[vb] = ((const T *)(f ? fp.b0[k] : fp.e0[k]))[node];
+#pragma unroll
+      for (int m = 1; m <= M; m++) {
+        const T *pc = (const T *)(f ? fp.bc[k][m - 1] : fp.ec[k][m - 1]);
+        val[vb + 2 * m - 1] = pc[2 * node];
+        val[vb + 2 * m] = pc[2 * node + 1];
+      }
+    }
+#pragma unroll
+  for (int v = 0; v < NV; v++) dst[v] = (v % NQ) ? (T)2 * val[v] : val[v];
+}
+
 template <typename T, int M>
 __global__ void __launch_bounds__(GS_NT)
     k_gather_sorted(const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
@@ -657,16 +682,15 @@ __global__ void __launch_bounds__(GS_NT)
                     const uint32_t *__restrict__ np_stay_p, T factor_push, int Nx, int Nr, T xmin,
                     T dx_inv, T rmin, T dr_inv, FieldPtrs fp) {
   typedef GatherCfg<T, M> Cfg;
-  constexpr int NV = Cfg::NV, NQ = Cfg::NQ, MAXC = Cfg::MAXC;
-  __shared__ T fs[MAXC * 4 * NV];
-  __shared__ int64_t cab[2];
+  constexpr int NV = Cfg::NV, MAXN = Cfg::MAXN;
+  __shared__ T fs[2 * MAXN * NV];
+  __shared__ int cab[4];
   const int64_t np_stay = (int64_t)(*np_stay_p);
   const int64_t s0 = (int64_t)blockIdx.x * GS_NT;
   if (s0 >= np_stay) return;  // uniform over the block
-  const int Nx_cell = Nx - 1;
   const int64_t slot = s0 + threadIdx.x;
   const bool live = slot < np_stay;
-  const int64_t nlive = (np_stay - s0 < GS_NT) ? np_stay - s0 : GS_NT;
+  const int nlive = (int)((np_stay - s0 < GS_NT) ? np_stay - s0 : GS_NT);
   uint32_t p = 0;
   T xp = 0, yp = 0, zp = 0, rp = 0;
   T u[3] = {0, 0, 0};
@@ -683,29 +707,29 @@ __global__ void __launch_bounds__(GS_NT)
     ix = (int)dfloor<T>((xp - xmin) * dx_inv);
     ir = (int)dfloor<T>((rp - rmin) * dr_inv);
   }
-  // slots below Np_stay hold particles of valid cells, in cell order
-  const int64_t cell = ix + (int64_t)ir * Nx_cell;
-  if (threadIdx.x == 0) cab[0] = cell;
-  if (threadIdx.x == nlive - 1) cab[1] = cell;
+  // slots below Np_stay hold particles of valid cells, in cell order: the
+  // first and the last live slot bound the block's cells
+  if (threadIdx.x == 0) {
+    cab[0] = ix;
+    cab[1] = ir;
+  }
+  if ((int)threadIdx.x == nlive - 1) {
+    cab[2] = ix;
+    cab[3] = ir;
+  }
   __syncthreads();
-  const int64_t cA = cab[0], ncl = cab[1] - cA + 1;
-  const bool staged = ncl <= MAXC;
-  if (staged) {
-    const int tot = (int)ncl * 4 * NV;
-    for (int e = threadIdx.x; e < tot; e += GS_NT) {
-      const int v = e / ((int)ncl * 4), rem = e - v * (int)ncl * 4;
-      const int cl = rem >> 2, corner = rem & 3;
-      const int64_t cc = cA + cl;
-      const int irc = (int)(cc / Nx_cell), ixc = (int)(cc - (int64_t)irc * Nx_cell);
-      const int64_t node = ixc + (corner & 1) + (int64_t)(irc + (corner >> 1)) * Nx;
-      const T val = field_val<T, M>(fp, v, node);
-      fs[(cl * 4 + corner) * NV + v] = (v % NQ) ? (T)2 * val : val;
-    }
+  const int ixA = cab[0], irA = cab[1];
+  const int ncol = cab[2] - ixA + 2;  // node columns under the block's cells
+  const bool staged = cab[3] == irA && ncol <= MAXN;
+  if (staged && (int)threadIdx.x < 2 * ncol) {
+    const int i = (int)threadIdx.x >= ncol ? 1 : 0;
+    const int64_t node = (int64_t)(ixA + (int)threadIdx.x - i * ncol) + (int64_t)(irA + i) * Nx;
+    stage_node<T, M>(fp, node, fs + threadIdx.x * NV);
   }
   __syncthreads();
   if (!live || !((int64_t)p < np_stay)) return;
   const int64_t igrid = ix + (int64_t)ir * Nx;
-  const int lc = (int)(cell - cA);
+  const int lc = ix - ixA;
   T dt_2 = (T)0.5 * factor_push;
   T rp_inv = (T)1. / rp;
   T sX1 = (xp - xmin) * dx_inv - (T)ix;
@@ -727,9 +751,9 @@ __global__ void __launch_bounds__(GS_NT)
   C[1][1] = sR1 * sX1;
   T e_p[3] = {0, 0, 0}, b_p[3] = {0, 0, 0};
   if (staged) {
-    const T *f0 = fs + lc * 4 * NV;
-    gather_sum<T, M, true>([&](int v, int i, int j) { return f0[(2 * i + j) * NV + v]; }, C, gr,
-                           gi, e_p, b_p);
+    const T *f0 = fs + lc * NV;
+    gather_sum<T, M, true>([&](int v, int i, int j) { return f0[(i * ncol + j) * NV + v]; }, C,
+                           gr, gi, e_p, b_p);
   } else {
     gather_sum<T, M, false>([&](int v, int i, int j) {
       return field_val<T, M>(fp, v, igrid + j + (int64_t)Nx * i);

@@ -10,8 +10,10 @@
 //      keys, i.e. 4M cells): per 4096-key tile an LDS digit histogram, one
 //      scan over [digit][tile], then a scatter that ranks keys inside the
 //      tile in index order (ballot-matched digits per wave, tagged per-wave
-//      counts in LDS); the last pass also writes the sorted keys
-//   3. cell_offset[c] = lower bound of c in the sorted keys, Np_stay
+//      counts in LDS); per-tile counts are stored one row per tile and
+//      scanned by chunks of tiles; the last pass also writes the sorted keys
+//   3. cell_offset[c] = lower bound of c in the sorted keys (from the slots
+//      where the sorted key changes), Np_stay
 // The result is exactly numpy.argsort(key, kind='stable') and the
 // reference's cell_offset; no global atomics, no order fix-up.
 #include "chm_common.h"
@@ -203,13 +205,35 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_final(uint32_t *__restrict__ i
 }
 
 // ------------------------------------------------------------------ offsets
-// cell_offset[c] = first sorted position whose key is >= c, c = 0..nbins
-// (lower bound by bisection over the sorted keys), and
+// cell_offset[c] = first sorted slot whose key is >= c, c = 0..nbins, and
 // Np_stay = cell_offset[nbins - 1], the start of the out-of-domain bin.
-__global__ void k_offsets_sorted(const uint32_t *__restrict__ skey, int64_t Np, int64_t nbins,
-                                 uint32_t *__restrict__ cell_offset, uint32_t *__restrict__ np_stay) {
+// Slot s starts the cells key[s-1] < c <= key[s] (key[-1] = -1,
+// key[Np] = nbins); these intervals partition 0..nbins.  A slot writes the
+// first OFF_RUN cells of its interval; k_offsets_gaps then finds the rest
+// (cells of long empty stretches, still holding the fill value) by bisection.
+#define OFF_RUN 32
+__global__ void k_offsets_fill(uint32_t *__restrict__ cell_offset, int64_t n) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c > nbins) return;
+  if (c < n) cell_offset[c] = 0xFFFFFFFFu;
+}
+
+__global__ void k_offsets_bounds(const uint32_t *__restrict__ skey, int64_t Np, int64_t nbins,
+                                 uint32_t *__restrict__ cell_offset, uint32_t *__restrict__ np_stay) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > Np) return;
+  const int64_t lo = s > 0 ? (int64_t)skey[s - 1] + 1 : 0;
+  int64_t hi = s < Np ? (int64_t)skey[s] : nbins;
+  if (hi > lo + OFF_RUN - 1) hi = lo + OFF_RUN - 1;
+  for (int64_t c = lo; c <= hi; c++) {
+    cell_offset[c] = (uint32_t)s;
+    if (c == nbins - 1) *np_stay = (uint32_t)s;
+  }
+}
+
+__global__ void k_offsets_gaps(const uint32_t *__restrict__ skey, int64_t Np, int64_t nbins,
+                               uint32_t *__restrict__ cell_offset, uint32_t *__restrict__ np_stay) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > nbins || cell_offset[c] != 0xFFFFFFFFu) return;
   int64_t lo = 0, hi = Np;
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
@@ -221,7 +245,8 @@ __global__ void k_offsets_sorted(const uint32_t *__restrict__ skey, int64_t Np, 
 }
 
 // ------------------------------------------------------------------ radix sort
-// Digits of nbits <= 2^RB buckets; counts[d * nblk + tile].
+// Digits of nbits <= 2^RB buckets; counts[tile * nd + d] (one contiguous
+// row per tile).
 template <int RB>
 __global__ void __launch_bounds__(RS_BS) k_rs_upsweep(const uint32_t *__restrict__ keys, int64_t n,
                                                       int shift, int nbits,
@@ -247,7 +272,7 @@ __global__ void __launch_bounds__(RS_BS) k_rs_upsweep(const uint32_t *__restrict
     run_add(h, act, (kk[i] >> shift) & mask, lane);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < nd; d += RS_BS) counts[(int64_t)d * nblk + blockIdx.x] = h[d];
+  for (int d = threadIdx.x; d < nd; d += RS_BS) counts[(int64_t)blockIdx.x * nd + d] = h[d];
 }
 
 // Stable scatter of one tile.  Items are visited in index order: round r,
@@ -269,7 +294,7 @@ __global__ void __launch_bounds__(RS_BS) k_rs_downsweep(const uint32_t *__restri
   const int nd = 1 << nbits;
   const uint32_t mask = (uint32_t)nd - 1u;
   for (int d = tid; d < nd; d += RS_BS) {
-    run[d] = offs[(int64_t)d * nblk + blockIdx.x];
+    run[d] = offs[(int64_t)blockIdx.x * nd + d];
     for (int w = 0; w < RS_BS / CHM_WAVE; w++) wc[w][d] = 0;
   }
   __syncthreads();
@@ -336,6 +361,51 @@ __global__ void __launch_bounds__(RS_BS) k_rs_downsweep(const uint32_t *__restri
   }
 }
 
+// Exclusive offsets of the per-tile digit counts, digit-major:
+// offs[t * nd + d] = sum of all counts of digits < d + counts of digit d in
+// tiles < t.  Tiles are grouped in chunks of RS_TC: k_rs_colsum writes the
+// chunk sums transposed (csumT[d * nch + ch]), a flat scan of csumT gives each
+// chunk's starting offset, and k_rs_coloffs runs through the chunk's tiles.
+#define RS_TC 16
+__global__ void __launch_bounds__(256) k_rs_colsum(const uint32_t *__restrict__ counts, int64_t nblk,
+                                                   int nbits, int64_t nch,
+                                                   uint32_t *__restrict__ csumT) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;  // ch * nd + d
+  const int nd = 1 << nbits;
+  if (e >= nch * nd) return;
+  const int64_t ch = e >> nbits;
+  const int d = (int)(e & (nd - 1));
+  const int64_t t0 = ch * RS_TC;
+  uint32_t v[RS_TC];
+#pragma unroll
+  for (int i = 0; i < RS_TC; i++) v[i] = t0 + i < nblk ? counts[(t0 + i) * nd + d] : 0u;
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < RS_TC; i++) s += v[i];
+  csumT[(int64_t)d * nch + ch] = s;
+}
+
+__global__ void __launch_bounds__(256) k_rs_coloffs(const uint32_t *__restrict__ counts, int64_t nblk,
+                                                    int nbits, int64_t nch,
+                                                    const uint32_t *__restrict__ cpre,
+                                                    uint32_t *__restrict__ offs) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int nd = 1 << nbits;
+  if (e >= nch * nd) return;
+  const int64_t ch = e >> nbits;
+  const int d = (int)(e & (nd - 1));
+  const int64_t t0 = ch * RS_TC;
+  uint32_t v[RS_TC];
+#pragma unroll
+  for (int i = 0; i < RS_TC; i++) v[i] = t0 + i < nblk ? counts[(t0 + i) * nd + d] : 0u;
+  uint32_t run = cpre[(int64_t)d * nch + ch];
+#pragma unroll
+  for (int i = 0; i < RS_TC; i++) {
+    if (t0 + i < nblk) offs[(t0 + i) * nd + d] = run;
+    run += v[i];
+  }
+}
+
 // Pass plan for keys of `bits` bits: one or two 8-bit passes up to 16 bits,
 // then two (up to 22 bits) or three passes of at most 11 bits.
 struct RsPlan {
@@ -367,18 +437,22 @@ static int key_bits(int64_t nbins) {
 
 // ------------------------------------------------------------------ driver
 struct SortWs {
-  uint32_t *counts, *offs, *bsum2, *kA, *kB, *vA, *vB;
-  int64_t nblk, ncounts, nscan2;
+  uint32_t *counts, *offs, *csum, *cpre, *bsum2, *kA, *kB, *vA, *vB;
+  int64_t nblk, nch;
 };
 
 static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
   int64_t nblk = (Np + RS_TILE - 1) / RS_TILE;
   if (nblk < 1) nblk = 1;
-  int64_t ncounts = ((int64_t)1 << rs_plan(key_bits(nbins)).rb) * nblk;
-  int64_t nscan2 = (ncounts + SCAN_TILE - 1) / SCAN_TILE;
+  const int64_t nch = (nblk + RS_TC - 1) / RS_TC;
+  const int64_t nd = (int64_t)1 << rs_plan(key_bits(nbins)).rb;
+  const int64_t ncounts = nd * nblk, ncsum = nd * nch;
+  const int64_t nscan2 = (ncsum + SCAN_TILE - 1) / SCAN_TILE;
   size_t off = 0;
   size_t o_counts = off; off += chm_align(ncounts * 4);
-  size_t o_offs = off; off += chm_align((ncounts + 1) * 4);
+  size_t o_offs = off; off += chm_align(ncounts * 4);
+  size_t o_csum = off; off += chm_align(ncsum * 4);
+  size_t o_cpre = off; off += chm_align((ncsum + 1) * 4);
   size_t o_bsum2 = off; off += chm_align(nscan2 * 4);
   size_t o_k = off; off += 2 * chm_align(Np * 4);
   size_t o_v = off; off += 2 * chm_align(Np * 4);
@@ -386,14 +460,15 @@ static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
     char *b = (char *)base;
     w->counts = (uint32_t *)(b + o_counts);
     w->offs = (uint32_t *)(b + o_offs);
+    w->csum = (uint32_t *)(b + o_csum);
+    w->cpre = (uint32_t *)(b + o_cpre);
     w->bsum2 = (uint32_t *)(b + o_bsum2);
     w->kA = (uint32_t *)(b + o_k);
     w->kB = (uint32_t *)(b + o_k + chm_align(Np * 4));
     w->vA = (uint32_t *)(b + o_v);
     w->vB = (uint32_t *)(b + o_v + chm_align(Np * 4));
     w->nblk = nblk;
-    w->ncounts = ncounts;
-    w->nscan2 = nscan2;
+    w->nch = nch;
   }
   return off;
 }
@@ -446,13 +521,16 @@ static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_
       const bool last = p == plan.npass - 1;
       uint32_t *kout = (p & 1) ? w.kB : w.kA;
       uint32_t *vout = last ? sort_indx : ((p & 1) ? w.vB : w.vA);
-      const int64_t nc = ((int64_t)1 << nb) * w.nblk;
-      const int64_t nsc = (nc + SCAN_TILE - 1) / SCAN_TILE;
+      const int64_t ncs = ((int64_t)1 << nb) * w.nch;
+      const int64_t nsc = (ncs + SCAN_TILE - 1) / SCAN_TILE;
       if (plan.rb == 8)
         k_rs_upsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
       else
         k_rs_upsweep<11><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
-      scan_u32(w.counts, nc, w.offs, w.bsum2, nsc, st);
+      k_rs_colsum<<<chm_blocks(ncs, 256), 256, 0, st>>>(w.counts, w.nblk, nb, w.nch, w.csum);
+      scan_u32(w.csum, ncs, w.cpre, w.bsum2, nsc, st);
+      k_rs_coloffs<<<chm_blocks(ncs, 256), 256, 0, st>>>(w.counts, w.nblk, nb, w.nch, w.cpre,
+                                                          w.offs);
       if (plan.rb == 8)
         k_rs_downsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs,
                                                               w.nblk, kout, vout);
@@ -465,7 +543,9 @@ static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_
     }
     skey = kin;
   }
-  k_offsets_sorted<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset, np_stay);
+  k_offsets_fill<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(cell_offset, nbins + 1);
+  k_offsets_bounds<<<chm_blocks(Np + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset, np_stay);
+  k_offsets_gaps<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset, np_stay);
   CHM_CHECK_LAUNCH();
   return CHM_OK;
 }

@@ -137,6 +137,36 @@ def test_sort_dense_cells_and_edges(env):
     assert e.Args["Np"] == 0
 
 
+@pytest.mark.parametrize("case", ["many_tiles", "sparse"])
+def test_sort_large_grid(env, case):
+    """A grid of 131,968 bins (18-bit keys, two radix passes): 300,000
+    particles span many chunks of sort tiles; a sparse species (a few
+    clusters, long stretches of empty cells) takes the gap search of the
+    offsets.  Keys, stable order, cell offsets and Np_stay match the oracle."""
+    g = env.tiny_grid(Nx=1024, Nr=130, M=1)
+    n = 300000 if case == "many_tiles" else 3000
+    solver, p, s, q = _pair_particles(env, g, n, 91, 1)
+    if case == "sparse":
+        d = p.DataDev
+        dx = (g["Xmax"] - g["Xmin"]) / (g["Nx"] - 1)
+        rng = np.random.default_rng(3)
+        centres = g["Xmin"] + dx * np.array([5.5, 300.2, 301.7, 900.4])
+        x = centres[rng.integers(0, 4, n)] + rng.uniform(-0.4, 0.4, n) * dx
+        d["x"].copy_(torch.from_numpy(x))
+        q.data["x"][:] = x
+    p.sort_parts(solver)
+    q.sort_parts(s)
+    key = p.DataDev["indx_in_cell"].cpu().numpy().view(np.uint32)
+    assert np.array_equal(key, q.data["indx_in_cell"])
+    assert np.array_equal(p.DataDev["sort_indx"].cpu().numpy().view(np.uint32),
+                          np.argsort(key, kind="stable"))
+    off = p.DataDev["cell_offset"].cpu().numpy().view(np.uint32)
+    assert np.array_equal(off, q.data["cell_offset"])
+    if case == "sparse":
+        assert np.max(np.diff(np.unique(key))) > 500  # long empty stretches
+    assert p.Args["Np_stay"] == q.Args["Np_stay"]
+
+
 def test_dens_profile_bit_exact(env):
     g = env.tiny_grid()
     solver, p, s, q = _pair_particles(env, g, 5000, 3, 1)
