@@ -1,7 +1,8 @@
 """Benchmark of the chimeraCL PIC step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C2]
-                    [--no-cpu] [--no-mixed] [--no-diff]
+                    [--no-cpu] [--no-mixed] [--no-diff] [--align-every K]
+                    [--aligned-section K]
 
 Workload (default): the medium laser-wakefield configuration, Nz=2048,
 Nr=256, modes m=0,1, 2x2x8 particles per cell (16.6M electrons plus as many
@@ -21,6 +22,12 @@ measured on the two differentiable workloads (Nz=1024, Nr=128, 100 steps,
 FP64, gradients against central finite differences; Nz=4096, Nr=256, 400
 steps, mixed, gradients against the FP64 adjoint), with the backward /
 forward time ratio.
+
+value follows the reference's loop exactly (particle storage never
+reordered).  The "aligned" section times both forms again with the electrons
+permuted into their sorted order every K steps (PIC_loop(align_every=K), an
+addition of this package): the same physics with another particle numbering,
+and the memory order of the particles kept close to their cells.
 
 Printed: one JSON line.
 """
@@ -54,6 +61,10 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--diag-steps", default="100,500")
+    ap.add_argument("--align-every", type=int, default=0,
+                    help="permute the electrons into their sorted order every K steps")
+    ap.add_argument("--aligned-section", type=int, default=20,
+                    help="also time both forms with align_every=K (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (HBM counter calibration)")
     return ap.parse_args()
@@ -76,7 +87,7 @@ def log(*a):
 
 
 class Sim:
-    def __init__(self, cfg_name, precision, with_oracle):
+    def __init__(self, cfg_name, precision, with_oracle, align_every=0):
         import chimeracl_amd as CA
         from chimeracl_amd import config as C
         from chimeracl_amd.methods.generic_methods_hip import Communicator
@@ -97,7 +108,8 @@ class Sim:
         if prof is not None:
             self.eons.dens_profile(*prof)
             self.ions.dens_profile(*prof)
-        self.loop = CA.PIC_loop(solvers=[self.solver], species=[self.eons, self.ions], graph=True)
+        self.loop = CA.PIC_loop(solvers=[self.solver], species=[self.eons, self.ions], graph=True,
+                                align_every=align_every)
         self.steps = 0
         self.orc = None
         if with_oracle:
@@ -110,7 +122,7 @@ class Sim:
             if prof is not None:
                 e.dens_profile(*prof)
                 i.dens_profile(*prof)
-            self.orc = (s, e, i, O.OracleLoop(s, [e, i]))
+            self.orc = (s, e, i, O.OracleLoop(s, [e, i], align_every=align_every))
 
     def step(self, n=1):
         for _ in range(n):
@@ -430,6 +442,35 @@ def diff_section():
     return out
 
 
+def aligned_section(args, k):
+    """Both forms again, with the electrons permuted into their sorted order
+    every k steps (PIC_loop(align_every=k)): same workload, warmup and timed
+    window as the main measurement."""
+    import torch
+    out = {"align_every": k}
+    first = min([int(v) for v in args.diag_steps.split(",") if v] or [0])
+    for name, prec in (("f64", "f64"), ("mixed", "mixed")):
+        if prec == "mixed" and args.no_mixed:
+            continue
+        sm = Sim(args.config, prec, False, k)
+        sm.step(max(args.warmup, first))
+        torch.cuda.synchronize()
+        t0 = time.time()
+        for _ in range(args.steps):
+            sm.loop.step()
+        torch.cuda.synchronize()
+        dt = time.time() - t0
+        Ne = sm.eons.Args["Np"]
+        out[name] = {"ms_per_step": dt / args.steps * 1e3, "value": Ne * args.steps / dt,
+                     "unit": "particle-steps/s", "steps": args.steps}
+        log(f"[bench] {name}, align_every={k}: {args.steps} steps in {dt:.3f}s: "
+            f"{dt / args.steps * 1e3:.3f} ms/step, {Ne * args.steps / dt:.4g} particle-steps/s")
+        del sm
+        gc.collect()
+        torch.cuda.empty_cache()
+    return out
+
+
 def _grad_errs(g, ref):
     return {"a0": abs(g[0] - ref[0]) / abs(ref[0]), "w0": abs(g[1] - ref[1]) / abs(ref[1]),
             "knots_rel_l2": float(np.linalg.norm(g[2:] - ref[2:]) / np.linalg.norm(ref[2:]))}
@@ -450,8 +491,8 @@ def main():
     cfg = C.CONFIGS[args.config]
     M = cfg["M"]
     want_cpu = not args.no_cpu
-    fp = Sim(args.config, "f64", want_cpu)
-    mx = None if args.no_mixed else Sim(args.config, "mixed", False)
+    fp = Sim(args.config, "f64", want_cpu, args.align_every)
+    mx = None if args.no_mixed else Sim(args.config, "mixed", False, args.align_every)
     sims = [s for s in (fp, mx) if s is not None]
     Ne = fp.eons.Args["Np"]
     add_div_scratch(fp.solver)
@@ -552,6 +593,7 @@ def main():
     del fp, mx, sims
     gc.collect()
     torch.cuda.empty_cache()
+    aligned = aligned_section(args, args.aligned_section) if args.aligned_section > 0 else None
     diff = None
     if not args.no_diff:
         t0 = time.time()
@@ -573,7 +615,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded uniform plasma, Gaussian laser)",
         "config": {"workload": f"{args.config}: " + cfg["desc"], "Nz": cfg["Nx"], "Nr": cfg["Nr"],
-                   "modes": M + 1, "ppc": list(cfg["ppc"]), "electrons": Ne, "dt": C.DZ},
+                   "modes": M + 1, "ppc": list(cfg["ppc"]), "electrons": Ne, "dt": C.DZ,
+                   "align_every": args.align_every},
         "fp64_oracle_error": errs,
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -581,6 +624,7 @@ def main():
         "diagnostics": diags,
         "kernels": kernels,
         "moving_window": window,
+        "aligned": aligned,
         "differentiable": diff,
     }
     print(json.dumps(out), flush=True)

@@ -65,6 +65,7 @@ _SIGS = {
     "chm_push_x_keys_f64": [P, P, P, P, P, P, P, f64, i64, R(Grid), P, P, sz, P],
     "chm_push_x_keys_f32": [P, P, P, P, P, P, P, f32, i64, R(Grid), P, P, sz, P],
     "chm_sort_from_keys": [i64, R(Grid), P, P, P, P, P, sz, P],
+    "chm_align_elems": [P, i64, P, P, i32, P],
     "chm_dens_profile_f64": [P, P, i64, P, P, P, i32, P],
     "chm_dens_profile_f32": [P, P, i64, P, P, P, i32, P],
     "chm_deposit_workspace_bytes": ([i32, i32, i32, i32], sz),
@@ -72,6 +73,10 @@ _SIGS = {
     "chm_deposit_scalar_f32": [P, P, P, P, P, P, i32, R(Grid), R(Planes), P, sz, P],
     "chm_deposit_vector_f64": [P, P, P, P, P, P, P, P, P, P, i32, R(Grid), R(Planes), P, sz, P],
     "chm_deposit_vector_f32": [P, P, P, P, P, P, P, P, P, P, i32, R(Grid), R(Planes), P, sz, P],
+    "chm_deposit_vector_push_f64": [P, P, P, P, P, P, P, P, P, P, i32, f64, i64, P, R(Grid),
+                                    R(Planes), P, sz, P],
+    "chm_deposit_vector_push_f32": [P, P, P, P, P, P, P, P, P, P, i32, f32, i64, P, R(Grid),
+                                    R(Planes), P, sz, P],
     "chm_deposit_add_staged_f64": [P, R(Grid), R(Planes), i32, P, sz, P],
     "chm_deposit_add_staged_f32": [P, R(Grid), R(Planes), i32, P, sz, P],
     "chm_postproc_deposit_f64": [R(Planes), i32, P, i32, i32, i32, P],

@@ -33,6 +33,23 @@ __device__ __forceinline__ typename cplx<T>::type cmul(typename cplx<T>::type a,
   return r;
 }
 
+__device__ __forceinline__ double dev_sqrt(double v) { return sqrt(v); }
+__device__ __forceinline__ float dev_sqrt(float v) { return sqrtf(v); }
+__device__ __forceinline__ double dev_floor(double v) { return floor(v); }
+__device__ __forceinline__ float dev_floor(float v) { return floorf(v); }
+
+// Cell key of index_and_sum_in_cell, particles_generic.cl:105-124: the cell
+// ix + ir*(Nx-1) of a valid cell, else the overflow bin (Nr-1)*(Nx-1).
+template <typename T>
+__device__ __forceinline__ uint32_t cell_key(T xp, T yp, T zp, int Nx_loc, int Nr_loc, T xmin,
+                                             T dx_inv, T rmin, T dr_inv) {
+  T r = dev_sqrt(yp * yp + zp * zp);
+  int ix = (int)dev_floor((xp - xmin) * dx_inv);
+  int ir = (int)dev_floor((r - rmin) * dr_inv);
+  if (ix > 0 && ix < Nx_loc - 1 && ir < Nr_loc - 1 && ir >= 0) return (uint32_t)(ix + ir * Nx_loc);
+  return (uint32_t)(Nr_loc * Nx_loc);
+}
+
 static inline unsigned chm_blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 static inline size_t chm_align(size_t n) { return (n + 255) & ~(size_t)255; }

@@ -43,6 +43,10 @@ struct PlanePtrs {
 //
 // KIND 0: rho (wp = w q); 1: J (wp = w g_inv q, values u); 2: field
 // adjoints of the gather (wp = 1, values in ux/uy/uz, modes weighted 2 e_m).
+// PUSH (KIND 1): the lane that stages a particle also applies the next
+// coordinate push (x += u dt g_inv, push_x's operations) and writes the
+// pushed position and its cell key (the keys of the following sort); the
+// sums use the positions before the push.
 template <int M, int KIND> struct DepCfg {
   static constexpr bool VEC = KIND != 0;
   static constexpr int NK = VEC ? 3 : 1;              // components
@@ -55,13 +59,14 @@ template <int M, int KIND> struct DepCfg {
 // spreads them over the LDS banks.
 #define DEP_SW(t) ((t) + ((t) >> 5))
 
-template <typename T, int M, int KIND>
+template <typename T, int M, int KIND, bool PUSH>
 __global__ void __launch_bounds__(64)
     k_cell_sums(const uint32_t *__restrict__ sort_indx, const uint32_t *__restrict__ cell_offset,
-                const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+                T *__restrict__ x, T *__restrict__ y, T *__restrict__ z,
                 const T *__restrict__ ux, const T *__restrict__ uy, const T *__restrict__ uz,
                 const T *__restrict__ g_inv, const T *__restrict__ w, T charge, int Nx, int Nr,
-                T xmin, T dx_inv, T rmin, T dr_inv, T *__restrict__ part) {
+                T xmin, T dx_inv, T rmin, T dr_inv, T *__restrict__ part, T dt,
+                uint32_t *__restrict__ key_out) {
   typedef DepCfg<M, KIND> Cfg;
   constexpr bool VEC = Cfg::VEC;
   constexpr int CAP = Cfg::CAP, NV = Cfg::NV, TPC = Cfg::TPC;
@@ -104,7 +109,17 @@ __global__ void __launch_bounds__(64)
         ly[rr] = y[p];
         lz[rr] = z[p];
         if (KIND == 1) {
-          lw[rr] = w[p] * g_inv[p] * charge;
+          const T gp = g_inv[p];
+          lw[rr] = w[p] * gp * charge;
+          if (PUSH) {
+            const T dt_g = dt * gp;
+            const T xn = lx[rr] + ux[p] * dt_g, yn = ly[rr] + uy[p] * dt_g,
+                    zn = lz[rr] + uz[p] * dt_g;
+            x[p] = xn;
+            y[p] = yn;
+            z[p] = zn;
+            key_out[p] = cell_key<T>(xn, yn, zn, Nx - 1, Nr - 1, xmin, dx_inv, rmin, dr_inv);
+          }
         } else if (KIND == 0) {
           lw[rr] = w[p] * charge;
         } else {
@@ -208,6 +223,30 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// The coordinate push + cell key of the particles a PUSH deposit does not
+// stage: sorted slots from Np_stay (= cell_offset[ncell], the start of the
+// out-of-domain bin) to Np.  Grid-stride over a fixed grid, as Np_stay is
+// known on the device only.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_push_keys_tail(const uint32_t *__restrict__ sort_indx, const uint32_t *__restrict__ cell_offset,
+                     int64_t ncell, int64_t Np, T *__restrict__ x, T *__restrict__ y,
+                     T *__restrict__ z, const T *__restrict__ ux, const T *__restrict__ uy,
+                     const T *__restrict__ uz, const T *__restrict__ g_inv, T dt, int Nx, int Nr,
+                     T xmin, T dx_inv, T rmin, T dr_inv, uint32_t *__restrict__ key_out) {
+  const int64_t np_stay = cell_offset[ncell];
+  for (int64_t slot = np_stay + (int64_t)blockIdx.x * 256 + threadIdx.x; slot < Np;
+       slot += (int64_t)gridDim.x * 256) {
+    const uint32_t p = sort_indx[slot];
+    const T dt_g = dt * g_inv[p];
+    const T xn = x[p] + ux[p] * dt_g, yn = y[p] + uy[p] * dt_g, zn = z[p] + uz[p] * dt_g;
+    x[p] = xn;
+    y[p] = yn;
+    z[p] = zn;
+    key_out[p] = cell_key<T>(xn, yn, zn, Nx - 1, Nr - 1, xmin, dx_inv, rmin, dr_inv);
+  }
+}
+
 // ------------------------------------------------------------------ pass 2
 template <typename T, int M, int NK>
 __global__ void __launch_bounds__(256)
@@ -265,16 +304,34 @@ extern "C" size_t chm_deposit_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M,
   return chm_align(nq * ncell * sizeof(double));
 }
 
+// Push of a PUSH deposit: time step and key output.
+template <typename T> struct DepPush {
+  T dt;
+  int64_t Np;
+  uint32_t *keys;  // nullptr: no push
+};
+
 template <typename T, int M, int KIND>
 static void launch_deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const T *x,
                            const T *y, const T *z, const T *ux, const T *uy, const T *uz,
                            const T *g_inv, const T *w, T charge, const chm_grid_t *g,
-                           const PlanePtrs &pp, T *part, bool pass1, hipStream_t st) {
+                           const PlanePtrs &pp, T *part, bool pass1, hipStream_t st,
+                           const DepPush<T> &push) {
   int64_t ncell = (int64_t)(g->Nx - 1) * (g->Nr - 1);
-  if (pass1)
-    k_cell_sums<T, M, KIND><<<chm_blocks(ncell, DepCfg<M, KIND>::CELLS), 64, 0, st>>>(
-      sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr, (T)g->xmin,
-      (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, part);
+  const unsigned nb = chm_blocks(ncell, DepCfg<M, KIND>::CELLS);
+  if (pass1 && KIND == 1 && push.keys) {
+    k_cell_sums<T, M, KIND, true><<<nb, 64, 0, st>>>(
+        sort_indx, cell_offset, (T *)x, (T *)y, (T *)z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr,
+        (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, part, push.dt, push.keys);
+    int64_t nt = chm_blocks(push.Np, 256);
+    k_push_keys_tail<T><<<(unsigned)(nt < 1024 ? (nt < 1 ? 1 : nt) : 1024), 256, 0, st>>>(
+        sort_indx, cell_offset, ncell, push.Np, (T *)x, (T *)y, (T *)z, ux, uy, uz, g_inv, push.dt,
+        g->Nx, g->Nr, (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, push.keys);
+  } else if (pass1) {
+    k_cell_sums<T, M, KIND, false><<<nb, 64, 0, st>>>(
+        sort_indx, cell_offset, (T *)x, (T *)y, (T *)z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr,
+        (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, part, (T)0, nullptr);
+  }
   int64_t nnode = (int64_t)(g->Nr - 1) * g->Nx;
   k_node_add<T, M, KIND ? 3 : 1><<<chm_blocks(nnode, 256), 256, 0, st>>>(cell_offset, part, g->Nx,
                                                                         g->Nr, pp);
@@ -284,7 +341,8 @@ template <typename T, int KIND>
 static int deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const T *x, const T *y,
                    const T *z, const T *ux, const T *uy, const T *uz, const T *g_inv, const T *w,
                    int32_t charge, const chm_grid_t *g, const chm_planes_t *flds, void *ws,
-                   size_t ws_bytes, void *stream, bool pass1 = true) {
+                   size_t ws_bytes, void *stream, bool pass1 = true,
+                   DepPush<T> push = DepPush<T>{(T)0, 0, nullptr}) {
   if (!g || !flds || g->M < 0 || g->M > CHM_MAXM || g->Nx < 4 || g->Nr < 3) return CHM_EINVAL;
   if (chm_deposit_workspace_bytes(g->Nx, g->Nr, g->M, KIND != 0) > ws_bytes || !ws) return CHM_EWORKSPACE;
   PlanePtrs pp = {};
@@ -300,7 +358,7 @@ static int deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const
 #define CASE(MM)                                                                                   \
   case MM:                                                                                         \
     launch_deposit<T, MM, KIND>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, q, g, pp,   \
-                               part, pass1, st);                                                   \
+                               part, pass1, st, push);                                             \
     break;
     CASE(0) CASE(1) CASE(2) CASE(3) CASE(4)
 #undef CASE
@@ -342,6 +400,38 @@ extern "C" int chm_deposit_vector_f32(const uint32_t *sort_indx, const uint32_t 
                                       size_t ws_bytes, void *stream) {
   return deposit<float, 1>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g, J,
                               ws, ws_bytes, stream);
+}
+
+template <typename T>
+static int deposit_vector_push(const uint32_t *sort_indx, const uint32_t *cell_offset, T *x, T *y,
+                               T *z, const T *ux, const T *uy, const T *uz, const T *g_inv,
+                               const T *w, int32_t charge, T dt, int64_t Np,
+                               uint32_t *indx_in_cell, const chm_grid_t *g, const chm_planes_t *J,
+                               void *ws, size_t ws_bytes, void *stream) {
+  if (!indx_in_cell || Np < 0 || Np > 0x7FFFFFF0ll) return CHM_EINVAL;
+  if (Np == 0) return CHM_OK;
+  return deposit<T, 1>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g, J, ws,
+                       ws_bytes, stream, true, DepPush<T>{dt, Np, indx_in_cell});
+}
+extern "C" int chm_deposit_vector_push_f64(const uint32_t *sort_indx, const uint32_t *cell_offset,
+                                           double *x, double *y, double *z, const double *ux,
+                                           const double *uy, const double *uz, const double *g_inv,
+                                           const double *w, int32_t charge, double dt, int64_t Np,
+                                           uint32_t *indx_in_cell, const chm_grid_t *g,
+                                           const chm_planes_t *J, void *ws, size_t ws_bytes,
+                                           void *stream) {
+  return deposit_vector_push<double>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge,
+                                     dt, Np, indx_in_cell, g, J, ws, ws_bytes, stream);
+}
+extern "C" int chm_deposit_vector_push_f32(const uint32_t *sort_indx, const uint32_t *cell_offset,
+                                           float *x, float *y, float *z, const float *ux,
+                                           const float *uy, const float *uz, const float *g_inv,
+                                           const float *w, int32_t charge, float dt, int64_t Np,
+                                           uint32_t *indx_in_cell, const chm_grid_t *g,
+                                           const chm_planes_t *J, void *ws, size_t ws_bytes,
+                                           void *stream) {
+  return deposit_vector_push<float>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge,
+                                    dt, Np, indx_in_cell, g, J, ws, ws_bytes, stream);
 }
 
 // Scatter of the gather's per-particle field adjoints (three components,

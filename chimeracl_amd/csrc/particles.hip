@@ -65,23 +65,34 @@ extern "C" int chm_push_x_f32(float *x, float *y, float *z, const float *px, con
   return push_x<float>(x, y, z, px, py, pz, g_inv, dt, Np, stream);
 }
 
-// ------------------------------------------------------------------ keys
-__device__ __forceinline__ double dev_sqrt(double v) { return sqrt(v); }
-__device__ __forceinline__ float dev_sqrt(float v) { return sqrtf(v); }
-__device__ __forceinline__ double dev_floor(double v) { return floor(v); }
-__device__ __forceinline__ float dev_floor(float v) { return floorf(v); }
-
-// Cell key of index_and_sum_in_cell, particles_generic.cl:105-124.
-template <typename T>
-__device__ __forceinline__ uint32_t cell_key(T xp, T yp, T zp, int Nx_loc, int Nr_loc, T xmin,
-                                             T dx_inv, T rmin, T dr_inv) {
-  T r = dev_sqrt(yp * yp + zp * zp);
-  int ix = (int)dev_floor((xp - xmin) * dx_inv);
-  int ir = (int)dev_floor((r - rmin) * dr_inv);
-  if (ix > 0 && ix < Nx_loc - 1 && ir < Nr_loc - 1 && ir >= 0) return (uint32_t)(ix + ir * Nx_loc);
-  return (uint32_t)(Nr_loc * Nx_loc);
+// ------------------------------------------------------------------ align
+// data_align_dbl / data_align_int, particles_generic.cl:156-186:
+// dst[i] = src[sort_indx[i]], i < n, for 8- or 4-byte elements.
+template <typename E>
+__global__ void __launch_bounds__(256) k_align(const uint32_t *__restrict__ idx, int64_t n,
+                                               const E *__restrict__ src, E *__restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
 }
 
+extern "C" int chm_align_elems(const uint32_t *sort_indx, int64_t n, const void *src, void *dst,
+                               int32_t elem_bytes, void *stream) {
+  if (n < 0 || (elem_bytes != 4 && elem_bytes != 8) || (n > 0 && (!sort_indx || !src || !dst)) ||
+      (src == dst && n > 0))
+    return CHM_EINVAL;
+  if (n == 0) return CHM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (elem_bytes == 8)
+    k_align<uint64_t><<<chm_blocks(n, 256), 256, 0, st>>>(sort_indx, n, (const uint64_t *)src,
+                                                          (uint64_t *)dst);
+  else
+    k_align<uint32_t><<<chm_blocks(n, 256), 256, 0, st>>>(sort_indx, n, (const uint32_t *)src,
+                                                          (uint32_t *)dst);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+// ------------------------------------------------------------------ keys
 // Run structure of a wave: lane is a head if its key differs from the lane
 // below.  Returns the head mask; `len` = run length for head lanes.
 __device__ __forceinline__ unsigned long long wave_runs(uint32_t rkey, int lane, int *len,
@@ -217,16 +228,38 @@ __global__ void k_offsets_fill(uint32_t *__restrict__ cell_offset, int64_t n) {
   if (c < n) cell_offset[c] = 0xFFFFFFFFu;
 }
 
-__global__ void k_offsets_bounds(const uint32_t *__restrict__ skey, int64_t Np, int64_t nbins,
-                                 uint32_t *__restrict__ cell_offset, uint32_t *__restrict__ np_stay) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s > Np) return;
-  const int64_t lo = s > 0 ? (int64_t)skey[s - 1] + 1 : 0;
-  int64_t hi = s < Np ? (int64_t)skey[s] : nbins;
-  if (hi > lo + OFF_RUN - 1) hi = lo + OFF_RUN - 1;
-  for (int64_t c = lo; c <= hi; c++) {
-    cell_offset[c] = (uint32_t)s;
-    if (c == nbins - 1) *np_stay = (uint32_t)s;
+// OFF_PER consecutive slots per lane (two 16-byte key loads).
+#define OFF_PER 8
+__global__ void __launch_bounds__(256) k_offsets_bounds(const uint32_t *__restrict__ skey, int64_t Np,
+                                                        int64_t nbins,
+                                                        uint32_t *__restrict__ cell_offset,
+                                                        uint32_t *__restrict__ np_stay) {
+  const int64_t s0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * OFF_PER;
+  if (s0 > Np) return;
+  uint32_t kv[OFF_PER];
+  if (s0 + OFF_PER <= Np) {
+    const uint4 a = *(const uint4 *)(skey + s0), b = *(const uint4 *)(skey + s0 + 4);
+    kv[0] = a.x; kv[1] = a.y; kv[2] = a.z; kv[3] = a.w;
+    kv[4] = b.x; kv[5] = b.y; kv[6] = b.z; kv[7] = b.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < OFF_PER; i++) kv[i] = s0 + i < Np ? skey[s0 + i] : 0u;
+  }
+  int64_t prev = s0 > 0 ? (int64_t)skey[s0 - 1] : -1;
+#pragma unroll
+  for (int i = 0; i < OFF_PER; i++) {
+    const int64_t s = s0 + i;
+    if (s > Np) break;
+    const int64_t cur = s < Np ? (int64_t)kv[i] : nbins;
+    if (cur > prev) {
+      int64_t hi = cur;
+      if (hi > prev + OFF_RUN) hi = prev + OFF_RUN;
+      for (int64_t c = prev + 1; c <= hi; c++) {
+        cell_offset[c] = (uint32_t)s;
+        if (c == nbins - 1) *np_stay = (uint32_t)s;
+      }
+    }
+    prev = cur;
   }
 }
 
@@ -275,88 +308,103 @@ __global__ void __launch_bounds__(RS_BS) k_rs_upsweep(const uint32_t *__restrict
   for (int d = threadIdx.x; d < nd; d += RS_BS) counts[(int64_t)blockIdx.x * nd + d] = h[d];
 }
 
-// Stable scatter of one tile.  Items are visited in index order: round r,
-// wave w, lane l holds item base + r*256 + 64*w + l.  Within a wave, lanes
-// with equal digits are matched by ballots; per-wave digit counts go to LDS
-// tagged with the round number (no clearing needed); run[d] is the tile's
-// running output position of digit d.
+// Lanes of the wave holding the same digit d (active lanes only).  Fast
+// paths: one digit in the wave, or non-decreasing digits (equal digits are
+// then contiguous); otherwise one ballot per digit bit.
+__device__ __forceinline__ unsigned long long match_peers(uint32_t d, bool act, int lane,
+                                                          int nbits) {
+  const unsigned long long amask = __ballot(act);
+  // active lanes are a prefix of the wave; lane 0 is active if any is
+  const uint32_t d0 = (uint32_t)__shfl((int)d, 0);
+  const uint32_t dprev = (uint32_t)__shfl_up((int)d, 1);
+  if (__ballot(act && d == d0) == amask) return amask;
+  if (__ballot(act && lane > 0 && dprev > d) == 0ull) {
+    const unsigned long long heads = __ballot(act && (lane == 0 || dprev != d));
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int hl = 63 - __clzll((long long)(heads & upto));
+    const unsigned long long above = heads & ~upto;
+    const int nx = above ? (__ffsll((long long)above) - 1) : 64;
+    const unsigned long long hi = (nx == 64) ? ~0ull : ((1ull << nx) - 1ull);
+    return hi & ~((1ull << hl) - 1ull) & amask;
+  }
+  unsigned long long peers = amask;
+  for (int b = 0; b < nbits; b++) {
+    const bool bit = (d >> b) & 1u;
+    const unsigned long long bal = __ballot(bit);
+    peers &= bit ? bal : ~bal;
+  }
+  return peers;
+}
+
+// Stable scatter of one tile.  Wave w owns the tile's items
+// w*RS_ITEMS*64 .. (w+1)*RS_ITEMS*64 - 1, visited in index order (round r,
+// lane l: item r*64 + l of the wave's range).  Pass 1: each wave counts its
+// digits (one LDS add per run of equal digits).  The waves' counts then
+// become their starting positions (tile offset + counts of the waves
+// before).  Pass 2: each item goes to its wave's running position of its
+// digit plus its rank among the equal digits of its round.  Two block
+// barriers per tile; within a wave, LDS operations take effect in order.
 template <int RB>
 __global__ void __launch_bounds__(RS_BS) k_rs_downsweep(const uint32_t *__restrict__ kin,
                                                         const uint32_t *__restrict__ vin, int64_t n,
                                                         int shift, int nbits,
-                                                        const uint32_t *__restrict__ offs, int64_t nblk,
+                                                        const uint32_t *__restrict__ offs,
                                                         uint32_t *__restrict__ kout,
                                                         uint32_t *__restrict__ vout) {
   constexpr int NB = 1 << RB;
-  __shared__ uint32_t run[NB];
-  __shared__ uint32_t wc[RS_BS / CHM_WAVE][NB];
+  constexpr int NW = RS_BS / CHM_WAVE;
+  __shared__ uint32_t hw[NW][NB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nd = 1 << nbits;
   const uint32_t mask = (uint32_t)nd - 1u;
-  for (int d = tid; d < nd; d += RS_BS) {
-    run[d] = offs[(int64_t)blockIdx.x * nd + d];
-    for (int w = 0; w < RS_BS / CHM_WAVE; w++) wc[w][d] = 0;
-  }
-  __syncthreads();
+  for (int d = tid; d < nd; d += RS_BS)
+#pragma unroll
+    for (int w = 0; w < NW; w++) hw[w][d] = 0;
   const unsigned long long lt = (1ull << lane) - 1ull;
-  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-  uint32_t kk[RS_ITEMS], vv[RS_ITEMS];
+  const int64_t wbase = (int64_t)blockIdx.x * RS_TILE + (int64_t)wave * (RS_ITEMS * CHM_WAVE);
+  uint32_t kk[RS_ITEMS], vv[RS_ITEMS], rc[RS_ITEMS];
 #pragma unroll
   for (int r = 0; r < RS_ITEMS; r++) {
-    const int64_t idx = base + (int64_t)r * RS_BS + tid;
+    const int64_t idx = wbase + r * CHM_WAVE + lane;
     const bool act = idx < n;
     kk[r] = act ? kin[idx] : 0u;
     vv[r] = act ? (vin ? vin[idx] : (uint32_t)idx) : 0u;
   }
+  __syncthreads();
+  uint32_t *h = hw[wave];
 #pragma unroll
   for (int r = 0; r < RS_ITEMS; r++) {
-    const int64_t idx = base + (int64_t)r * RS_BS + tid;
-    const bool act = idx < n;
-    const uint32_t k = kk[r], v = vv[r];
-    uint32_t d = (k >> shift) & mask;
-    const unsigned long long amask = __ballot(act);
-    unsigned long long peers;
-    // active lanes are a prefix of the wave; lane 0 is active if any is
-    const uint32_t d0 = (uint32_t)__shfl((int)d, 0);
-    const uint32_t dprev = (uint32_t)__shfl_up((int)d, 1);
-    if (__ballot(act && d == d0) == amask) {
-      peers = amask;  // one digit in the wave
-    } else if (__ballot(act && lane > 0 && dprev > d) == 0ull) {
-      // non-decreasing digits: equal digits form contiguous runs
-      const unsigned long long heads = __ballot(act && (lane == 0 || dprev != d));
-      const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-      const int hl = 63 - __clzll((long long)(heads & upto));
-      const unsigned long long above = heads & ~upto;
-      const int nx = above ? (__ffsll((long long)above) - 1) : 64;
-      const unsigned long long hi = (nx == 64) ? ~0ull : ((1ull << nx) - 1ull);
-      peers = hi & ~((1ull << hl) - 1ull) & amask;
-    } else {
-      peers = amask;
-      for (int b = 0; b < nbits; b++) {
-        bool bit = (d >> b) & 1u;
-        unsigned long long bal = __ballot(bit);
-        peers &= bit ? bal : ~bal;
-      }
-    }
+    const bool act = wbase + r * CHM_WAVE + lane < n;
+    const uint32_t d = (kk[r] >> shift) & mask;
+    const unsigned long long peers = match_peers(d, act, lane, nbits);
     const uint32_t rank = (uint32_t)__popcll((long long)(peers & lt));
     const uint32_t cnt = (uint32_t)__popcll((long long)peers);
-    const bool leader = act && rank == 0;
-    const uint32_t tag = (uint32_t)(r + 1) << 16;
-    if (leader) wc[wave][d] = tag | cnt;
-    __syncthreads();
-    uint32_t dest = 0;
-    if (act) {
-      dest = run[d] + rank;
-      for (int w = 0; w < wave; w++) {
-        uint32_t e = wc[w][d];
-        if ((e & 0xFFFF0000u) == tag) dest += e & 0xFFFFu;
-      }
+    rc[r] = rank | (cnt << 8);
+    if (act && rank == 0) atomicAdd(&h[d], cnt);
+  }
+  __syncthreads();
+  for (int d = tid; d < nd; d += RS_BS) {
+    uint32_t run = offs[(int64_t)blockIdx.x * nd + d];
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+      const uint32_t c = hw[w][d];
+      hw[w][d] = run;
+      run += c;
     }
-    __syncthreads();
-    if (leader) atomicAdd(&run[d], cnt);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; r++) {
+    const bool act = wbase + r * CHM_WAVE + lane < n;
+    const uint32_t d = (kk[r] >> shift) & mask;
+    const uint32_t rank = rc[r] & 0xFFu, cnt = rc[r] >> 8;
+    uint32_t dest = 0;
+    if (act) dest = h[d] + rank;
+    __builtin_amdgcn_wave_barrier();
+    if (act && rank == 0) atomicAdd(&h[d], cnt);
     if (act) {
-      if (kout) kout[dest] = k;
-      vout[dest] = v;
+      if (kout) kout[dest] = kk[r];
+      vout[dest] = vv[r];
     }
   }
 }
@@ -533,10 +581,10 @@ static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_
                                                           w.offs);
       if (plan.rb == 8)
         k_rs_downsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs,
-                                                              w.nblk, kout, vout);
+                                                              kout, vout);
       else
         k_rs_downsweep<11><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, vin, Np, shift, nb, w.offs,
-                                                               w.nblk, kout, vout);
+                                                               kout, vout);
       kin = kout;
       vin = vout;
       shift += nb;
@@ -544,7 +592,8 @@ static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_
     skey = kin;
   }
   k_offsets_fill<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(cell_offset, nbins + 1);
-  k_offsets_bounds<<<chm_blocks(Np + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset, np_stay);
+  k_offsets_bounds<<<chm_blocks(Np / OFF_PER + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset,
+                                                                       np_stay);
   k_offsets_gaps<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset, np_stay);
   CHM_CHECK_LAUNCH();
   return CHM_OK;

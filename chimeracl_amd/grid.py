@@ -28,8 +28,10 @@ class Grid(GridMethodsHIP):
             self.depose_scalar(parts, "w", "rho", charge=parts.Args["charge"])
         self.postproc_depose_scalar("rho")
 
-    def depose_currents(self, species=[]):
-        """grid.py:33-54: J of every mobile species."""
+    def depose_currents(self, species=[], push_half=False):
+        """grid.py:33-54: J of every mobile species.  With push_half, each
+        mobile species is also pushed by dt/2 and keyed on this grid in the
+        same pass, as push_coords('half', grid=self) would do next."""
         for m in range(self.Args["M"] + 1):
             for c in self.Args["vec_comps"]:
                 self.set_to(self.DataDev["J" + c + "_m" + str(m)], 0)
@@ -37,7 +39,11 @@ class Grid(GridMethodsHIP):
         for parts in species:
             if "Immobile" in parts.Args:
                 continue
-            self.depose_vector(parts, momenta, ["g_inv", "w"], "J", charge=parts.Args["charge"])
+            if push_half:
+                self.depose_vector_push(parts, momenta, ["g_inv", "w"], "J",
+                                        charge=parts.Args["charge"], dt=parts.Args["dt_2"])
+            else:
+                self.depose_vector(parts, momenta, ["g_inv", "w"], "J", charge=parts.Args["charge"])
         self.postproc_depose_vector("J")
 
     def gather_and_push(self, species=[]):

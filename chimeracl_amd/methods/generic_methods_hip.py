@@ -103,6 +103,10 @@ class GenericMethodsHIP:
         """Calls the precision-suffixed entry point on the current stream."""
         _lib.call(name + self.comm.sfx, *args, self.comm.stream())
 
+    def call_raw(self, name, *args):
+        """Calls an entry point without a precision suffix on the current stream."""
+        _lib.call(name, *args, self.comm.stream())
+
     def dev_arr(self, val=None, shape=(1,), dtype=np.double):
         """generic_methods_cl.py:78-87: host ndarray -> device, 0 -> zeros,
         None -> uninitialised, scalar -> filled."""

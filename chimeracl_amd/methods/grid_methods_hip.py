@@ -103,6 +103,27 @@ class GridMethodsHIP(GenericMethodsHIP):
                   ptr(d[factors[1]]), int(charge), self._grid(),
                   self._planes_arr([vec_fld + c for c in comps]), ptr(ws), ws.numel())
 
+    def depose_vector_push(self, parts, vec, factors, vec_fld, charge, dt):
+        """depose_vector, then the species' next coordinate push by dt and
+        the cell keys of its next sort on this grid, in one pass (the
+        push_coords + key pass that follow depose_currents in the step,
+        pic_loop.py:82-91); positions and keys equal those of push_coords
+        with grid=self."""
+        if parts.Args["Np"] <= 0:
+            return
+        if not parts.flag_sorted:
+            parts.sort_parts(self)
+        parts._ensure_sort_buffers(self)
+        d, ws = parts.DataDev, self._dep_ws()
+        comps = self.Args["vec_comps"]
+        self.call("chm_deposit_vector_push", ptr(d["sort_indx"]), ptr(d["cell_offset"]), ptr(d["x"]),
+                  ptr(d["y"]), ptr(d["z"]), *(ptr(d[v]) for v in vec), ptr(d[factors[0]]),
+                  ptr(d[factors[1]]), int(charge), dt, parts.Args["Np"], ptr(d["indx_in_cell"]),
+                  self._grid(), self._planes_arr([vec_fld + c for c in comps]), ptr(ws), ws.numel())
+        parts._keys_for = self
+        parts.flag_sorted = False
+        parts.version += 1
+
     def _postproc(self, names):
         a = self.Args
         self.call("chm_postproc_deposit", self._planes_arr(names), len(names),

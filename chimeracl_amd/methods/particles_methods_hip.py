@@ -1,9 +1,9 @@
 """Particle methods (replaces chimeraCL/methods/particles_methods_cl.py).
 
-push_coords, index_sort, dens_profile and the particle bookkeeping call the
-HIP kernels of libchimera_hip.so.  Random plasma/beam generation on the
-device (make_new_domain / make_new_beam, Threefry) is not part of this
-build: particles come in through add_particles(source=...), e.g. from
+push_coords, index_sort, dens_profile, align and the particle bookkeeping
+call the HIP kernels of libchimera_hip.so; make_new_domain / make_new_beam
+fill new particles on the device from a counter-based generator.  Particles
+can also come in through add_particles(source=...), e.g. from
 chimeracl_amd.config.load_plasma.
 """
 from __future__ import annotations
@@ -26,6 +26,7 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         self.set_global_working_group_size()
         self._sort_ws = None
         self._keys_for = None
+        self._align_buf = None
         # bumped whenever positions, weights, momenta or the order change
         self.version = 0
 
@@ -205,11 +206,40 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         """particles_methods_cl.py:263-293: keep the Np_stay particles of the
         valid cells, in sorted order (used by the moving window)."""
         Np_stay = self.Args["Np_stay"]
-        idx = self.DataDev["sort_indx"][:Np_stay].long()
+        srt = self.DataDev["sort_indx"]
         for comp in comps_align:
-            self.DataDev[comp] = self.DataDev[comp].index_select(0, idx).contiguous()
+            src = self.DataDev[comp]
+            dst = torch.empty(Np_stay, dtype=src.dtype, device=src.device)
+            self.call_raw("chm_align_elems", ptr(srt), Np_stay, ptr(src), ptr(dst), src.element_size())
+            self.DataDev[comp] = dst
         self.reset_num_parts()
         self.flag_sorted = False
+        self.version += 1
+
+    def reorder_parts(self):
+        """Storage order only, no counterpart in the reference: every
+        component of all Np particles is permuted into the current sorted
+        order with the align kernel (none dropped), and sort_indx becomes the
+        identity, so the particles' memory order follows their cells again.
+        Values, cell offsets and Np_stay are unchanged; particle indices
+        change.  Needs a current sort (flag_sorted)."""
+        Np = self.Args["Np"]
+        if Np == 0 or not self.flag_sorted:
+            return
+        d = self.DataDev
+        if self._align_buf is None or self._align_buf[0].numel() != Np:
+            dev = d["x"].device
+            self._align_buf = (torch.empty(Np, dtype=torch.float64, device=dev),
+                               torch.arange(Np, dtype=torch.int32, device=dev))
+        scr, iota = self._align_buf
+        srt = d["sort_indx"]
+        for comp in self._arg_names():
+            a = d[comp]
+            tmp = scr.view(torch.uint8)[: a.numel() * a.element_size()].view(a.dtype)
+            self.call_raw("chm_align_elems", ptr(srt), Np, ptr(a), ptr(tmp), a.element_size())
+            a.copy_(tmp)
+        srt.copy_(iota)
+        self._keys_for = None
         self.version += 1
 
     def free_mp(self):

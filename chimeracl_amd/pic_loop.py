@@ -3,6 +3,8 @@
 The order of operations is the reference's.  Adjacent reference calls are
 fused, with identical arithmetic:
   * push_coords + the cell keys of the following sort_parts -> one pass;
+  * with one solver, the J deposit + the second push_coords + its keys ->
+    one pass;
   * fb_transform(dir=0) of rho, J + fields_smooth -> filter applied in the
     transform's last pass;
   * the dN0 <- dN1 copy + field_grad -> one assembly kernel;
@@ -25,7 +27,13 @@ loop_steps = ["frame", "push-x", "sort", "depose", "transform", "smooth", "data_
 
 
 class PIC_loop:
-    def __init__(self, solvers=[], species=[], frames=[], diags=[], timit=False, graph=False):
+    def __init__(self, solvers=[], species=[], frames=[], diags=[], timit=False, graph=False,
+                 align_every=0):
+        """align_every > 0 (an addition, off by default): every align_every
+        steps, at the start of the step, the mobile species are permuted into
+        their current sorted order (Particles.reorder_parts), so that their
+        memory order keeps following the cells."""
+        self.align_every = align_every
         self.solvers = solvers
         self.mainsolver = self.solvers[0]
         self.species = species
@@ -72,14 +80,18 @@ class PIC_loop:
             self.timer_start()
             parts.sort_parts(grid=self.mainsolver)
             self.timer_record("sort")
+        # one solver: its J deposit also applies the second half push and
+        # the sort keys (the same positions and keys as push_coords)
+        fused = len(self.solvers) == 1
         self.timer_start()
         for solver in self.solvers:
-            solver.depose_currents(species=self.species)
+            solver.depose_currents(species=self.species, push_half=fused)
         self.timer_record("depose")
         for parts in self.species:
-            self.timer_start()
-            parts.push_coords(mode="half", grid=self.mainsolver)
-            self.timer_record("push-x")
+            if not fused:
+                self.timer_start()
+                parts.push_coords(mode="half", grid=self.mainsolver)
+                self.timer_record("push-x")
             self.timer_start()
             parts.sort_parts(grid=self.mainsolver)
             self.timer_record("sort")
@@ -109,6 +121,10 @@ class PIC_loop:
     def step(self):
         for diag in self.diags:
             diag.make_record(self.it)
+        if self.align_every and self.it > 0 and self.it % self.align_every == 0:
+            for parts in self.species:
+                if "Immobile" not in parts.Args:
+                    parts.reorder_parts()
         if self.graph and self._graph is None and self.it >= 1 and not self.frames and not self.timit:
             self._capture()
         if self._graph is not None:

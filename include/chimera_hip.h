@@ -197,6 +197,12 @@ int chm_sort_from_keys(int64_t Np, const chm_grid_t *g,
                        const uint32_t *indx_in_cell, uint32_t *cell_offset,
                        uint32_t *sort_indx, uint32_t *np_stay, void *ws,
                        size_t ws_bytes, void *stream);
+/* data_align_dbl / data_align_int (kernels/particles_generic.cl:156-186,
+ * ParticleMethodsCL.align_and_damp, methods/particles_methods_cl.py:263-286):
+ * dst[i] = src[sort_indx[i]] for i < n; elem_bytes 8 (double) or 4 (float,
+ * uint32).  dst must not be src. */
+int chm_align_elems(const uint32_t *sort_indx, int64_t n, const void *src,
+                    void *dst, int32_t elem_bytes, void *stream);
 int chm_dens_profile_f64(const double *x, double *w, int64_t Np,
                          const double *xx, const double *ff,
                          const double *dxm1, int32_t nknots, void *stream);
@@ -235,6 +241,31 @@ int chm_deposit_vector_f32(const uint32_t *sort_indx,
                            const float *g_inv, const float *w, int32_t charge,
                            const chm_grid_t *g, const chm_planes_t *J,
                            void *ws, size_t ws_bytes, void *stream);
+/* chm_deposit_vector_* followed by the step's next coordinate push and the
+ * cell keys of its sort (pic_loop.py:82-91: depose_currents, then
+ * push_coords('half') and the key pass of sort_parts), in one pass over the
+ * particles: J from the positions before the push, then every one of the Np
+ * particles is pushed by dt (x += u*dt*g_inv, chm_push_x_*'s operations)
+ * and indx_in_cell receives the keys chm_push_x_keys_* would write, ready
+ * for chm_sort_from_keys.  sort_indx / cell_offset must be the current
+ * sort of these particles on grid g. */
+int chm_deposit_vector_push_f64(const uint32_t *sort_indx,
+                                const uint32_t *cell_offset, double *x,
+                                double *y, double *z, const double *ux,
+                                const double *uy, const double *uz,
+                                const double *g_inv, const double *w,
+                                int32_t charge, double dt, int64_t Np,
+                                uint32_t *indx_in_cell, const chm_grid_t *g,
+                                const chm_planes_t *J, void *ws,
+                                size_t ws_bytes, void *stream);
+int chm_deposit_vector_push_f32(const uint32_t *sort_indx,
+                                const uint32_t *cell_offset, float *x, float *y,
+                                float *z, const float *ux, const float *uy,
+                                const float *uz, const float *g_inv,
+                                const float *w, int32_t charge, float dt,
+                                int64_t Np, uint32_t *indx_in_cell,
+                                const chm_grid_t *g, const chm_planes_t *J,
+                                void *ws, size_t ws_bytes, void *stream);
 /* The node pass of a deposit alone: adds to the planes the per-cell sums a
  * previous chm_deposit_scalar_* (vector = 0) or chm_deposit_vector_*
  * (vector = 1) call left in ws, in the same order.  For a species whose

@@ -390,6 +390,18 @@ class OracleParticles:
         self.Args["Np"] = self.Args["Np_stay"] = idx.size
         self.flag_sorted = False
 
+    def reorder_parts(self):
+        """The storage reorder of the product (not in the reference): every
+        component of all Np particles permuted by sort_indx with the
+        data_align gather (particles_generic.cl:156-186), none dropped;
+        sort_indx becomes the identity, cell offsets and Np_stay stay."""
+        if self.Args["Np"] == 0 or not self.flag_sorted:
+            return
+        idx = self.data["sort_indx"].astype(np.int64)
+        for k in self._keys():
+            self.data[k] = np.ascontiguousarray(self.data[k][idx])
+        self.data["sort_indx"] = np.arange(self.Args["Np"], dtype=np.uint32)
+
     def push_coords(self, mode="half"):
         """push_coords, particles_methods_cl.py:206-223."""
         if self.Args["Np"] == 0 or self.immobile:
@@ -767,11 +779,16 @@ class OracleFrame:
 class OracleLoop:
     """PIC_loop.step, pic_loop.py:57-142 (frames, no diagnostics)."""
 
-    def __init__(self, solver, species, frames=()):
+    def __init__(self, solver, species, frames=(), align_every=0):
         self.solver, self.species, self.frames, self.it = solver, species, list(frames), 0
+        self.align_every = align_every
 
     def step(self):
         s = self.solver
+        if self.align_every and self.it > 0 and self.it % self.align_every == 0:
+            for p in self.species:
+                if not p.immobile:
+                    p.reorder_parts()
         for fr in self.frames:
             if np.mod(self.it, fr.Args["Steps"]) == 0:
                 fr.shift_grids(grids=[s])

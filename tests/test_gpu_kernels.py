@@ -87,6 +87,38 @@ def test_push_sort_deposit_bit_exact(env, gi):
 
 
 @pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_deposit_push_fused_matches_separate(env, gi):
+    """depose_currents(push_half=True) equals depose_currents, then
+    push_coords('half', grid), bit for bit: J, the pushed positions (also of
+    the particles outside the domain), the keys and the next sort."""
+    import chimeracl_amd as CA
+    from chimeracl_amd import config as C
+    g = env.tiny_grid(**GRIDS[gi])
+    solver, p, s, q = _pair_particles(env, g, 30000, 40 + gi, g["M"])
+    arr = _rand_particles(g, 30000, 40 + gi)
+    p2 = CA.Particles({"Nppc": (2, 2, 4), "dx": solver.Args["dx"], "dr": solver.Args["dr"],
+                       "dt": solver.Args["dt"], "dens": 0.002, "charge": -1}, solver.comm)
+    p2.add_particles(source=C.ParticleSource(arr))
+    out = {}
+    for tag, parts, fused in (("sep", p, False), ("fused", p2, True)):
+        parts.sort_parts(solver)
+        solver.depose_currents(species=[parts], push_half=fused)
+        if not fused:
+            parts.push_coords(mode="half", grid=solver)
+        parts.sort_parts(solver)
+        d = parts.DataDev
+        out[tag] = {n: env.field_host(solver, n).copy() for n in env.real_names(g["M"], flds=("J",))}
+        for k in ("x", "y", "z"):
+            out[tag][k] = parts.host(k)
+        for k in ("indx_in_cell", "sort_indx", "cell_offset"):
+            out[tag][k] = d[k].cpu().numpy().view(np.uint32).copy()
+        out[tag]["Np_stay"] = parts.Args["Np_stay"]
+    assert out["sep"]["Np_stay"] < 30000  # some particles outside the domain
+    for k, v in out["sep"].items():
+        assert np.array_equal(v, out["fused"][k]), k
+
+
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
 def test_gather_push_bit_exact(env, gi):
     g = env.tiny_grid(**GRIDS[gi])
     solver, p, s, q = _pair_particles(env, g, 20000, 20 + gi, g["M"])
