@@ -23,6 +23,14 @@ FP64, gradients against central finite differences; Nz=4096, Nr=256, 400
 steps, mixed, gradients against the FP64 adjoint), with the backward /
 forward time ratio.
 
+roofline: the gather + push (the dominant particle kernel), its algorithmic
+bytes over its mean duration in the timed steps.  HIP graphs here cannot hold
+event nodes, so the state before the timed steps is saved, the timed steps
+run (graph replay, wall clock), the state is restored and the same steps are
+rerun eagerly with HIP events around the gather in each step; the rerun must
+end bit-identical to the timed run.  traffic: HBM bytes per launch from the
+rocprofv3 counter summary in profiles/.
+
 value follows the reference's loop exactly (particle storage never
 reordered).  The "aligned" section times both forms again with the electrons
 permuted into their sorted order every K steps (PIC_loop(align_every=K), an
@@ -67,6 +75,9 @@ def parse():
                     help="also time both forms with align_every=K (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (HBM counter calibration)")
+    ap.add_argument("--gather-only", type=int, default=0, metavar="STEPS",
+                    help="only run STEPS eager steps of both forms (no graphs), then three "
+                         "gather + push launches of each (for counter collection)")
     return ap.parse_args()
 
 
@@ -80,6 +91,25 @@ def calibrate_copy():
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"calibrate_copy_bytes": a.numel() * 4, "launches": 3}), flush=True)
+
+
+def gather_only(args):
+    """Both forms of the workload, `args.gather_only` eager PIC steps each
+    (no HIP graph), then three gather + push launches per form."""
+    import torch
+    out = {}
+    for prec in ("f64", "mixed"):
+        sm = Sim(args.config, prec, False)
+        sm.loop.graph = False
+        sm.step(args.gather_only)
+        for _ in range(3):
+            sm.solver._gather_and_push(sm.eons, ["E", "B"])
+        torch.cuda.synchronize()
+        out[prec] = {"steps": args.gather_only, "gather_launches": 3, "electrons": sm.eons.Args["Np"]}
+        del sm
+        gc.collect()
+        torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
 
 
 def log(*a):
@@ -235,8 +265,8 @@ def rho_copy(sim):
 
 # ---------------------------------------------------------------- kernels
 def kernel_table(sim, reps, w):
-    """HIP-event timings (on the stream the kernels run on) of the step's
-    phases, with their algorithmic HBM bytes (or DHT flops) per call."""
+    """HIP-event timings (burst_time, on the stream the kernels run on) of the
+    step's phases, with their algorithmic HBM bytes (or DHT flops) per call."""
     import torch
     s, e = sim.solver, sim.eons
     a = s.Args
@@ -244,17 +274,9 @@ def kernel_table(sim, reps, w):
     Nk = Nr - 1
     r1 = Nr * Nx * w * (1 + 2 * M)          # one real-space component, all modes
     s1 = 2 * w * Nk * Nx * (M + 1)          # one spectral component, all modes
-    st = torch.cuda.current_stream()
 
     def timeit(fn):
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-        fn()
-        for x, y in ev:
-            x.record(st)
-            fn()
-            y.record(st)
-        torch.cuda.synchronize()
-        return float(np.mean([x.elapsed_time(y) for x, y in ev])) * 1e-3
+        return burst_time(fn, reps)
 
     rows = {}
 
@@ -304,25 +326,105 @@ def pmc_traffic(config, form):
     return None
 
 
-def gather_roofline(sim, reps, w):
-    """The dominant kernel (gather + Boris push), timed with HIP events on the
-    stream it runs on.  Algorithmic bytes per launch: each electron reads x,
-    y, z, px, py, pz and writes px, py, pz, g_inv (10 words)."""
+def burst_time(fn, reps):
+    """Mean GPU time of fn: reps calls captured back to back in a HIP graph,
+    replayed between two HIP events on the stream it runs on (no host gaps
+    between the launches)."""
+    import torch
+    fn()  # buffers exist before the capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    cap.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(g, stream=cap):
+            for _ in range(reps):
+                fn()
+    torch.cuda.current_stream().wait_stream(cap)
+    st = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g.replay()  # first replay uploads the graph
+    a.record(st)
+    g.replay()
+    b.record(st)
+    torch.cuda.synchronize()
+    del g
+    return a.elapsed_time(b) * 1e-3 / reps
+
+
+def snapshot(sim):
+    """(tensor, copy) pairs of everything the step reads and writes: the
+    solver's and the species' device data."""
+    import torch
+    out, seen = [], set()
+    for owner in (sim.solver, sim.eons, sim.ions):
+        for v in owner.DataDev.values():
+            if isinstance(v, torch.Tensor) and v.is_cuda and id(v) not in seen:
+                seen.add(id(v))
+                out.append((v, v.clone()))
+    return out
+
+
+def restore(snap):
+    for live, saved in snap:
+        live.copy_(saved)
+
+
+def fingerprint(sim):
+    """Copies of the fields and electron data that a step leaves behind."""
+    d, e = sim.solver.DataDev, sim.eons.DataDev
+    names = [k for k in d if k[0] in "EBJ" and "_m" in k and "_fb" not in k]
+    return [d[k].clone() for k in sorted(names)] + \
+        [e[k].clone() for k in ("x", "y", "z", "px", "py", "pz", "g_inv", "sort_indx")]
+
+
+def replay_timed_gather(sim, steps):
+    """Runs `steps` PIC steps eagerly (the kernels the graph replays, in the
+    same order) with a pair of HIP events around the electrons' gather + push
+    in every step, on the stream it is launched on; returns the mean, min and
+    max of its duration (ms).  The host enqueues ahead of the GPU, so each
+    pair brackets the kernel alone."""
     import torch
     s, e = sim.solver, sim.eons
-    st = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-    for a, b in ev:
-        a.record(st)
-        s._gather_and_push(e, ["E", "B"])
-        b.record(st)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    cur = [0]
+    orig = s._gather_and_push
+
+    def timed(parts, flds):
+        if parts is e:
+            ev[cur[0]][0].record()
+        orig(parts, flds)
+        if parts is e:
+            ev[cur[0]][1].record()
+
+    s._gather_and_push = timed
+    try:
+        for i in range(steps):
+            cur[0] = i
+            sim.loop._body()
+            sim.loop.it += 1
+    finally:
+        del s._gather_and_push
     torch.cuda.synchronize()
-    t = float(np.mean([a.elapsed_time(b) for a, b in ev])) * 1e-3
-    nbytes = e.Args["Np"] * 10 * w
+    ms = [a.elapsed_time(b) for a, b in ev]
+    return {"mean_ms": float(np.mean(ms)), "min_ms": float(np.min(ms)), "max_ms": float(np.max(ms)),
+            "steps": steps}
+
+
+def window_roofline(sim, gm, w):
+    """Roofline of the gather + push over the timed steps: algorithmic bytes
+    (10 words per electron) over its mean duration in those steps."""
+    t = gm["mean_ms"] * 1e-3
+    nbytes = sim.eons.Args["Np"] * 10 * w
     achieved = nbytes / t / 1e9
     return {"kernel": "k_gather_push", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "avg_launch_us": t * 1e6, "algorithmic_bytes": nbytes}
+            "avg_launch_us": t * 1e6, "algorithmic_bytes": nbytes,
+            "measured": f"HIP events around the kernel in each of the {gm['steps']} timed steps "
+                        f"(an eager rerun of the same steps from the saved state), min "
+                        f"{gm['min_ms'] * 1e3:.0f} us, max {gm['max_ms'] * 1e3:.0f} us, rerun "
+                        f"bit-identical to the timed run: {gm['replay_identical']}"}
 
 
 def moving_window(sim, events=4, steps_per_event=20):
@@ -482,6 +584,9 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.gather_only:
+        gather_only(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in "
                           "DESIGN.md but not implemented", "n_gpus": args.gpus}))
@@ -554,18 +659,35 @@ def main():
         sm.step(args.warmup)
     first_diag = min(diag_at) if diag_at else 0
     advance_to(max(first_diag, fp.steps))
-    timing = {}
+    timing, gather_ms = {}, {}
     for name, sm in (("f64", fp), ("mixed", mx)):
         if sm is None:
             continue
+        snap = snapshot(sm)
+        it0 = sm.loop.it
         torch.cuda.synchronize()
         t0 = time.time()
         for _ in range(args.steps):
             sm.loop.step()
         torch.cuda.synchronize()
         dt = time.time() - t0
-        sm.steps += args.steps
         timing[name] = dt
+        # the same steps again from the saved state, in a graph with HIP
+        # event nodes around the electron gather: its duration in every
+        # timed step (the step is deterministic, so the replay ends in the
+        # same state, which is checked)
+        after = fingerprint(sm)
+        restore(snap)
+        del snap
+        sm.loop.it = it0
+        gather_ms[name] = replay_timed_gather(sm, args.steps)
+        same = fingerprint(sm)
+        ident = all(torch.equal(x, y) for x, y in zip(after, same))
+        gather_ms[name]["replay_identical"] = ident
+        log(f"[bench] {name}: eager replay of the timed steps identical: {ident}; "
+            f"gather {gather_ms[name]}")
+        del after, same
+        sm.steps += args.steps
         log(f"[bench] {name}: {args.steps} steps in {dt:.3f}s: {dt / args.steps * 1e3:.3f} ms/step, "
             f"{Ne * args.steps / dt:.4g} particle-steps/s")
     if mx is not None and mx.steps != fp.steps:
@@ -576,7 +698,7 @@ def main():
     dt = timing["f64"]
     ms = dt / args.steps * 1e3
     pps = Ne * args.steps / dt
-    roof = gather_roofline(fp, args.kernel_reps, 8)
+    roof = window_roofline(fp, gather_ms["f64"], 8)
     roof["traffic"] = pmc_traffic(args.config, "f64")
     mixed = None
     if mx is not None:
@@ -584,7 +706,7 @@ def main():
         mixed = {"value": Ne * args.steps / dtm, "unit": "particle-steps/s",
                  "ms_per_step": dtm / args.steps * 1e3, "pic_steps_per_s": args.steps / dtm,
                  "speedup_vs_f64": dt / dtm, "dtype": "f32 particles/FFT, split-bf16x3 DHT",
-                 "gather_roofline": gather_roofline(mx, args.kernel_reps, 4)}
+                 "gather_roofline": window_roofline(mx, gather_ms["mixed"], 4)}
         mixed["gather_roofline"]["traffic"] = pmc_traffic(args.config, "mixed")
     kernels = {"f64": kernel_table(fp, args.kernel_reps, 8)}
     if mx is not None:

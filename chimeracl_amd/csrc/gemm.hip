@@ -218,10 +218,28 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
       }
 }
 
-int chm_gemm_launch(const GemmBatch &gb, int nitems, int kind, hipStream_t st) {
+int chm_gemm_launch(const GemmBatch &gb_in, int nitems, int kind, hipStream_t st) {
   if (nitems <= 0) return CHM_OK;
-  if (nitems > GEMM_MAXB || gb.M <= 0 || gb.K <= 0 || gb.N <= 0) return CHM_EINVAL;
-  if (gb.lda < gemm_lda(gb.K)) return CHM_EINVAL;
+  if (nitems > GEMM_MAXB || gb_in.M <= 0 || gb_in.K <= 0 || gb_in.N <= 0) return CHM_EINVAL;
+  if (gb_in.lda < gemm_lda(gb_in.K)) return CHM_EINVAL;
+  // A complex B operand taken as it is (alpha = 1, no second operand, no
+  // m = -1 map) is a real [K][2 Nx] matrix: read it directly.
+  const GemmBatch *pgb = &gb_in;
+  GemmBatch plain;
+  if (gb_in.bmode == 1 && gb_in.N == 2 * gb_in.Nx) {
+    bool all = true;
+    for (int i = 0; i < nitems && all; i++) {
+      const GemmItem &it = gb_in.it[i];
+      all = it.alpha.x == 1.0 && it.alpha.y == 0.0 && !it.Y && !it.remapX;
+    }
+    if (all) {
+      plain = gb_in;
+      plain.bmode = 0;
+      plain.ldb = 2 * gb_in.Nx;
+      pgb = &plain;
+    }
+  }
+  const GemmBatch &gb = *pgb;
   dim3 grid((gb.N + BN - 1) / BN, (gb.M + BM - 1) / BM, nitems);
   if (kind == GEMM_F64) k_gemm<double><<<grid, 256, 0, st>>>(gb);
   else if (kind == GEMM_F32) k_gemm<float><<<grid, 256, 0, st>>>(gb);
