@@ -21,7 +21,10 @@ bounded sample timed on this host's cores.  The differentiable form is
 measured on the two differentiable workloads (Nz=1024, Nr=128, 100 steps,
 FP64, gradients against central finite differences; Nz=4096, Nr=256, 400
 steps, mixed, gradients against the FP64 adjoint), with the backward /
-forward time ratio.
+forward time ratio.  The small (C1, FP64 vs the CPU oracle after 5 steps;
+mixed vs FP64 at steps 100 and 200) and large (C3, M=2 with a density ramp;
+mixed vs FP64 at steps 100 and 1000) configurations report the same errors
+and charge residuals ("configs").
 
 roofline: the gather + push (the dominant particle kernel), its algorithmic
 bytes over its mean duration in the timed steps.  HIP graphs here cannot hold
@@ -75,6 +78,8 @@ def parse():
                     help="also time both forms with align_every=K (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (HBM counter calibration)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the error report of the small (C1) and large (C3) configurations")
     ap.add_argument("--gather-only", type=int, default=0, metavar="STEPS",
                     help="only run STEPS eager steps of both forms (no graphs), then three "
                          "gather + push launches of each (for counter collection)")
@@ -573,6 +578,58 @@ def aligned_section(args, k):
     return out
 
 
+def config_section(name, diag_at, oracle_steps=0):
+    """Configuration `name` in both forms, in lockstep: at the steps in
+    diag_at, the mixed run's relative L2 errors against the FP64 run (E, B,
+    J, momenta), both charge residuals and energies; with oracle_steps, the
+    FP64 run against the CPU FP64 oracle after that many steps."""
+    import torch
+    from chimeracl_amd import config as C
+    t_all = time.time()
+    fp = Sim(name, "f64", oracle_steps > 0)
+    mx = Sim(name, "mixed", False)
+    M = fp.solver.Args["M"]
+    add_div_scratch(fp.solver)
+    cfg = C.CONFIGS[name]
+    out = {"workload": f"{name}: " + cfg["desc"], "Nz": cfg["Nx"], "Nr": cfg["Nr"], "modes": M + 1,
+           "ppc": list(cfg["ppc"]), "electrons": fp.eons.Args["Np"], "steps": max(diag_at),
+           "profile": cfg.get("profile")}
+    targets = sorted(set(diag_at) | ({oracle_steps} if oracle_steps else set()))
+    t_gpu = 0.0
+    for t in targets:
+        torch.cuda.synchronize()
+        t0 = time.time()
+        for sm in (fp, mx):
+            sm.step(t - 1 - sm.steps)
+        torch.cuda.synchronize()
+        t_gpu += time.time() - t0
+        prev = [rho_copy(sm) for sm in (fp, mx)]
+        for sm in (fp, mx):
+            sm.step(1)
+        torch.cuda.synchronize()
+        if t == oracle_steps:
+            t0 = time.time()
+            oloop = fp.orc[3]
+            for _ in range(oracle_steps):
+                oloop.step()
+            out["fp64_vs_oracle"] = {"step": t, "rel_l2": oracle_errors(fp, M),
+                                     "oracle_s": time.time() - t0}
+            fp.orc = None
+        if t in diag_at:
+            out[f"step_{t}"] = {
+                "fp64": {"charge_residual": charge_residual(fp.solver, fp, prev[0]),
+                         "energy": energies(fp)},
+                "mixed": {"charge_residual": charge_residual(fp.solver, mx, prev[1]),
+                          "energy": energies(mx), "rel_l2_vs_fp64": mixed_errors(mx, fp, M)}}
+        log(f"[bench] {name} step {t}: {out.get(f'step_{t}', out.get('fp64_vs_oracle'))}")
+    out["gpu_seconds_both_forms"] = t_gpu
+    out["section_seconds"] = time.time() - t_all
+    del fp, mx
+    gc.collect()
+    torch.cuda.empty_cache()
+    return out
+
+
 def _grad_errs(g, ref):
     return {"a0": abs(g[0] - ref[0]) / abs(ref[0]), "w0": abs(g[1] - ref[1]) / abs(ref[1]),
             "knots_rel_l2": float(np.linalg.norm(g[2:] - ref[2:]) / np.linalg.norm(ref[2:]))}
@@ -716,6 +773,10 @@ def main():
     gc.collect()
     torch.cuda.empty_cache()
     aligned = aligned_section(args, args.aligned_section) if args.aligned_section > 0 else None
+    configs = None
+    if not args.no_configs and args.config == "C2":
+        configs = {"C1": config_section("C1", [100, 200], oracle_steps=0 if args.no_cpu else 5),
+                   "C3": config_section("C3", [100, 1000])}
     diff = None
     if not args.no_diff:
         t0 = time.time()
@@ -747,6 +808,7 @@ def main():
         "kernels": kernels,
         "moving_window": window,
         "aligned": aligned,
+        "configs": configs,
         "differentiable": diff,
     }
     print(json.dumps(out), flush=True)
