@@ -117,6 +117,26 @@ def gather_only(args):
     print(json.dumps(out), flush=True)
 
 
+def host_info():
+    """CPU model, usable cores and the thread settings of the oracle."""
+    info = {"cores_available": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        cfg = np.show_config(mode="dicts")
+        blas = cfg.get("Build Dependencies", {}).get("blas", {})
+        info["blas"] = f"{blas.get('name')} {blas.get('version')}"
+    except Exception:  # numpy without the dict mode
+        info["blas"] = None
+    return info
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -494,6 +514,22 @@ def diff_section():
         gc.collect()
         torch.cuda.empty_cache()
 
+    def field_state(sim, a0, R, kv, n):
+        """The forward run's final E, B, J planes and electron momenta at the
+        nominal parameters (FP64 copies)."""
+        with torch.no_grad():
+            sim.forward(a0, R, kv, n)
+        torch.cuda.synchronize()
+        d, e = sim.solver.DataDev, sim.eons.DataDev
+        M = sim.solver.Args["M"]
+        st = {f: [d[f"{f}{c}_m{m}"][1:].to(torch.complex128 if m else torch.float64).clone()
+                  for c in "xyz" for m in range(M + 1)] for f in ("E", "B", "J")}
+        st["p"] = [e[k].double().clone() for k in ("px", "py", "pz")]
+        return st
+
+    def state_errs(a, ref):
+        return {k: _rel(list(zip(a[k], ref[k]))) for k in ("E", "B", "J", "p")}
+
     gam = lambda outs, rg: A.objective_gamma2(*outs[:3], threshold=5.0, tau=0.25)
     wake = lambda outs, rg: A.objective_wake_energy(list(outs[3:]), rg, ncols=256)
 
@@ -523,11 +559,15 @@ def diff_section():
         "knots_significant": int(big.sum()),
         "knots": len(kn_fd)}
     out["C4_f64"] = res
+    ref_state = field_state(sim, a0, R, kv, n)
     free(sim)
     sim, a0, R, kv, n = build("C4", "mixed")
     gmx, res = grads(sim, a0, R, kv, n, gam)
     res["grad_err_vs_fp64_adjoint"] = _grad_errs(gmx, g64)
+    res["fields_vs_fp64_forward"] = {"step": n, "rel_l2": state_errs(field_state(sim, a0, R, kv, n),
+                                                                     ref_state)}
     out["C4_mixed"] = res
+    del ref_state
     free(sim)
 
     # C5: the mixed adjoint against the FP64 adjoint, after 50 steps and
@@ -537,6 +577,7 @@ def diff_section():
     sim, a0, R, kv, n = build("C5", "f64")
     for ns in (50, n):
         ref[ns] = grads(sim, a0, R, kv, ns, wake)
+    ref_state = field_state(sim, a0, R, kv, n)
     free(sim)
     sim, a0, R, kv, n = build("C5", "mixed")
     for ns in (50, n):
@@ -545,6 +586,9 @@ def diff_section():
         res["fp64"] = {k: ref[ns][1][k] for k in ("objective", "forward_s", "backward_s",
                                                    "backward_over_forward")}
         out[f"C5_mixed_{ns}steps"] = res
+    out[f"C5_mixed_{n}steps"]["fields_vs_fp64_forward"] = {
+        "step": n, "rel_l2": state_errs(field_state(sim, a0, R, kv, n), ref_state)}
+    del ref_state
     free(sim)
     return out
 
@@ -676,6 +720,8 @@ def main():
         errs = oracle_errors(fp, M)
         cpu = {"value": Ne * args.cpu_steps / tc, "unit": "particle-steps/s", "cores": nthreads,
                "kind": "port",
+               "phases_s_per_step": {k: v / args.cpu_steps for k, v in oloop.Timer.items()},
+               "host": host_info(),
                "sample": f"{args.cpu_steps} full PIC steps of the same workload ({Ne} electrons, "
                          f"{fp.g['Nx']}x{fp.g['Nr']}, M={M}) by the CPU FP64 oracle "
                          f"(C/OpenMP particle loops, numpy FFT/BLAS), {tc:.1f}s"}

@@ -47,6 +47,8 @@ import ctypes
 import os
 import subprocess
 
+import time
+
 import numpy as np
 from scipy.special import jn_zeros, jv
 
@@ -492,7 +494,9 @@ class OracleSolver:
             else:
                 buf = np.fft.ifft(f[fb] * phs, axis=1)
                 if m == 0:
-                    buf = buf.real
+                    # contiguous, so that the product below runs in BLAS (a
+                    # strided .real view takes numpy's slow generic loop)
+                    buf = np.ascontiguousarray(buf.real)
                 if mode == "full":
                     buf = self.dht["DHT_inv"][m] @ buf
                 f[rs][1:] = buf
@@ -782,9 +786,20 @@ class OracleLoop:
     def __init__(self, solver, species, frames=(), align_every=0):
         self.solver, self.species, self.frames, self.it = solver, species, list(frames), 0
         self.align_every = align_every
+        # wall time per phase, keyed by the reference's loop_steps names
+        # (pic_loop.py:5-8)
+        self.Timer = dict.fromkeys(["frame", "push-x", "sort", "depose", "transform", "smooth",
+                                    "data_copy", "grad", "push-eb", "damp-eb", "restore_B",
+                                    "gather + push-p"], 0.0)
+
+    def _t(self, name, t0):
+        t1 = time.perf_counter()
+        self.Timer[name] += t1 - t0
+        return t1
 
     def step(self):
         s = self.solver
+        t = time.perf_counter()
         if self.align_every and self.it > 0 and self.it % self.align_every == 0:
             for p in self.species:
                 if not p.immobile:
@@ -793,24 +808,40 @@ class OracleLoop:
             if np.mod(self.it, fr.Args["Steps"]) == 0:
                 fr.shift_grids(grids=[s])
                 fr.inject_plasma(self.species, s)
+        t = self._t("frame", t)
         for p in self.species:
             p.push_coords("half")
+            t = self._t("push-x", t)
             p.sort_parts(s)
+            t = self._t("sort", t)
         s.depose_currents(self.species)
+        t = self._t("depose", t)
         for p in self.species:
             p.push_coords("half")
+            t = self._t("push-x", t)
             p.sort_parts(s)
+            t = self._t("sort", t)
         s.depose_charge(self.species)
+        t = self._t("depose", t)
         s.fb_transform(scals=["rho"], vects=["J"], dir=0)
+        t = self._t("transform", t)
         s.fields_smooth(["rho", "Jx", "Jy", "Jz"])
+        t = self._t("smooth", t)
         for m in range(s.M + 1):
             for c in s.comps:
                 s.f[f"dN0{c}_fb_m{m}"][:] = s.f[f"dN1{c}_fb_m{m}"]
+        t = self._t("data_copy", t)
         s.field_grad("rho", "dN1")
+        t = self._t("grad", t)
         s.push_fields()
+        t = self._t("push-eb", t)
         s.damp_fields()
+        t = self._t("damp-eb", t)
         s.restore_B_fb()
+        t = self._t("restore_B", t)
         s.fb_transform(vects=["E", "B"], dir=1)
+        t = self._t("transform", t)
         s.gather_and_push(self.species)
+        self._t("gather + push-p", t)
         self.it += 1
         return self.it
