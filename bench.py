@@ -16,8 +16,8 @@ mixed form (FP32 particles and FFT, split-BF16 3-term DHT products); the
 two runs advance in lockstep so that the mixed form's E, B, J and momentum
 errors and both charge-conservation residuals are measured against the FP64
 run at steps 100 and 500.  The FP64 run's error against the CPU FP64 oracle
-is measured after the first steps, with the oracle's throughput on that
-bounded sample timed on this host's cores.  The differentiable form is
+is measured after the first 12 steps, with the oracle's throughput on 10 of
+them (a bounded sample) timed on this host's cores.  The differentiable form is
 measured on the two differentiable workloads (Nz=1024, Nr=128, 100 steps,
 FP64, gradients against central finite differences; Nz=4096, Nr=256, 400
 steps, mixed, gradients against the FP64 adjoint), with the backward /
@@ -69,7 +69,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-warmup", type=int, default=2)
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--diag-steps", default="100,500")
     ap.add_argument("--align-every", type=int, default=0,
@@ -704,27 +705,34 @@ def main():
     add_div_scratch(fp.solver)
     torch.cuda.synchronize()
 
-    # ---- CPU baseline (oracle) on a bounded sample: the first cpu_steps steps
+    # ---- CPU baseline (oracle) on a bounded sample: cpu_warmup untimed
+    # steps, then cpu_steps timed steps; the FP64 GPU run is compared with it
+    # after the same steps
     cpu = errs = None
     if want_cpu:
         from oracle import oracle as O
         nthreads = O.lib().orc_num_threads()
         s, e, i, oloop = fp.orc
+        for _ in range(args.cpu_warmup):
+            oloop.step()
+        oloop.Timer = dict.fromkeys(oloop.Timer, 0.0)
         t0 = time.time()
         for _ in range(args.cpu_steps):
             oloop.step()
         tc = time.time() - t0
+        nst = args.cpu_warmup + args.cpu_steps
         for sm in sims:
-            sm.step(args.cpu_steps)  # eager first step, captured graph from the second
+            sm.step(nst)  # eager first step, captured graph from the second
         torch.cuda.synchronize()
         errs = oracle_errors(fp, M)
+        errs["after_steps"] = nst
         cpu = {"value": Ne * args.cpu_steps / tc, "unit": "particle-steps/s", "cores": nthreads,
                "kind": "port",
                "phases_s_per_step": {k: v / args.cpu_steps for k, v in oloop.Timer.items()},
                "host": host_info(),
-               "sample": f"{args.cpu_steps} full PIC steps of the same workload ({Ne} electrons, "
-                         f"{fp.g['Nx']}x{fp.g['Nr']}, M={M}) by the CPU FP64 oracle "
-                         f"(C/OpenMP particle loops, numpy FFT/BLAS), {tc:.1f}s"}
+               "sample": f"{args.cpu_steps} full PIC steps (after {args.cpu_warmup} untimed) of the "
+                         f"same workload ({Ne} electrons, {fp.g['Nx']}x{fp.g['Nr']}, M={M}) by the "
+                         f"CPU FP64 oracle (C/OpenMP particle loops, numpy FFT/BLAS), {tc:.1f}s"}
         log(f"[bench] oracle {args.cpu_steps} steps in {tc:.1f}s on {nthreads} threads; errors {errs}")
         fp.orc = None
         del s, e, i, oloop
