@@ -24,6 +24,17 @@ template <typename T> __device__ __forceinline__ T dfloor(T v);
 template <> __device__ __forceinline__ double dfloor<double>(double v) { return floor(v); }
 template <> __device__ __forceinline__ float dfloor<float>(float v) { return floorf(v); }
 
+// a*b + c.  FP64: a product and a sum, each rounded (the reference's
+// operations in its order, bit-exact against the oracle).  FP32 (the mixed
+// form, compared with FP64 to a tolerance): one fused multiply-add.
+template <typename T> __device__ __forceinline__ T madd(T a, T b, T c);
+template <> __device__ __forceinline__ double madd<double>(double a, double b, double c) {
+  return a * b + c;
+}
+template <> __device__ __forceinline__ float madd<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+
 struct PlanePtrs {
   void *m0[3];
   void *mc[3][CHM_MAXM];
@@ -194,18 +205,18 @@ __global__ void __launch_bounds__(64)
         if (VEC) {
           const T u = st[VU + k][t];
           const T jproj = C * u;
-          a0 += C * u;
+          a0 = madd(C, u, a0);
 #pragma unroll
           for (int m = 1; m <= M; m++) {
-            ar[m - 1] += jproj * st[VE + 2 * (m - 1)][t];
-            ai[m - 1] += jproj * st[VE + 2 * (m - 1) + 1][t];
+            ar[m - 1] = madd(jproj, st[VE + 2 * (m - 1)][t], ar[m - 1]);
+            ai[m - 1] = madd(jproj, st[VE + 2 * (m - 1) + 1][t], ai[m - 1]);
           }
         } else {
           a0 += C;
 #pragma unroll
           for (int m = 1; m <= M; m++) {
-            ar[m - 1] += C * st[VE + 2 * (m - 1)][t];
-            ai[m - 1] += C * st[VE + 2 * (m - 1) + 1][t];
+            ar[m - 1] = madd(C, st[VE + 2 * (m - 1)][t], ar[m - 1]);
+            ai[m - 1] = madd(C, st[VE + 2 * (m - 1) + 1][t], ai[m - 1]);
           }
         }
       }
@@ -590,6 +601,31 @@ struct FieldPtrs {
   const void *ec[3][CHM_MAXM], *bc[3][CHM_MAXM];
 };
 
+// Relativistic Boris rotation, grid_deposit_m1.cl:472-507, in the
+// reference's operation order (madd keeps it exact in FP64).  Updates u,
+// returns the new 1/gamma.
+template <typename T>
+__device__ __forceinline__ T boris(T (&u)[3], const T (&e_p)[3], const T (&b_p)[3], T dt_2) {
+  T um[3], up[3], u0[3], t[3], s[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) um[k] = madd(dt_2, e_p[k], u[k]);
+  T g = (T)1. / dsqrt<T>(madd(um[2], um[2], madd(um[1], um[1], madd(um[0], um[0], (T)1.))));
+#pragma unroll
+  for (int k = 0; k < 3; k++) t[k] = dt_2 * b_p[k] * g;
+  const T t2 = (T)2. / madd(t[2], t[2], madd(t[1], t[1], madd(t[0], t[0], (T)1.)));
+#pragma unroll
+  for (int k = 0; k < 3; k++) s[k] = t[k] * t2;
+  u0[0] = madd(-um[2], t[1], madd(um[1], t[2], um[0]));
+  u0[1] = madd(um[2], t[0], madd(-um[0], t[2], um[1]));
+  u0[2] = madd(-um[1], t[0], madd(um[0], t[1], um[2]));
+  up[0] = madd(-u0[2], s[1], madd(u0[1], s[2], um[0]));
+  up[1] = madd(u0[2], s[0], madd(-u0[0], s[2], um[1]));
+  up[2] = madd(-u0[1], s[0], madd(u0[0], s[1], um[2]));
+#pragma unroll
+  for (int k = 0; k < 3; k++) u[k] = madd(dt_2, e_p[k], up[k]);
+  return (T)1. / dsqrt<T>(madd(u[2], u[2], madd(u[1], u[1], madd(u[0], u[0], (T)1.))));
+}
+
 // gather_and_push, grid_deposit_m1.cl:330-511 (m0: grid_deposit_m0.cl:280-427).
 // One lane per particle index p; the reference's per-slot launch visits the
 // same set {p < Np_stay, p in a valid cell} and each particle's update only
@@ -640,39 +676,21 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int j = 0; j < 2; j++) {
         int64_t id = igrid + j + (int64_t)Nx * i;
-        e_p[k] += C[i][j] * ((const T *)fp.e0[k])[id];
-        b_p[k] += C[i][j] * ((const T *)fp.b0[k])[id];
+        e_p[k] = madd(C[i][j], ((const T *)fp.e0[k])[id], e_p[k]);
+        b_p[k] = madd(C[i][j], ((const T *)fp.b0[k])[id], b_p[k]);
 #pragma unroll
         for (int m = 1; m <= M; m++) {
           const T *e = (const T *)fp.ec[k][m - 1];
           const T *b = (const T *)fp.bc[k][m - 1];
           T er = 2 * e[2 * id], ei = 2 * e[2 * id + 1];
           T br = 2 * b[2 * id], bi = 2 * b[2 * id + 1];
-          e_p[k] += C[i][j] * er * gr[m];
-          e_p[k] -= C[i][j] * ei * gi[m];
-          b_p[k] += C[i][j] * br * gr[m];
-          b_p[k] -= C[i][j] * bi * gi[m];
+          e_p[k] = madd(C[i][j] * er, gr[m], e_p[k]);
+          e_p[k] = madd(-(C[i][j] * ei), gi[m], e_p[k]);
+          b_p[k] = madd(C[i][j] * br, gr[m], b_p[k]);
+          b_p[k] = madd(-(C[i][j] * bi), gi[m], b_p[k]);
         }
       }
-  // Boris rotation, grid_deposit_m1.cl:472-507
-  T um[3], up[3], u0[3], t[3], s[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) um[k] = u[k] + dt_2 * e_p[k];
-  T g_p_inv = (T)1. / dsqrt<T>((T)1. + um[0] * um[0] + um[1] * um[1] + um[2] * um[2]);
-#pragma unroll
-  for (int k = 0; k < 3; k++) t[k] = dt_2 * b_p[k] * g_p_inv;
-  T t2 = (T)2. / ((T)1. + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-#pragma unroll
-  for (int k = 0; k < 3; k++) s[k] = t[k] * t2;
-  u0[0] = um[0] + um[1] * t[2] - um[2] * t[1];
-  u0[1] = um[1] - um[0] * t[2] + um[2] * t[0];
-  u0[2] = um[2] + um[0] * t[1] - um[1] * t[0];
-  up[0] = um[0] + u0[1] * s[2] - u0[2] * s[1];
-  up[1] = um[1] - u0[0] * s[2] + u0[2] * s[0];
-  up[2] = um[2] + u0[0] * s[1] - u0[1] * s[0];
-#pragma unroll
-  for (int k = 0; k < 3; k++) u[k] = up[k] + dt_2 * e_p[k];
-  g_p_inv = (T)1. / dsqrt<T>((T)1. + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  const T g_p_inv = boris<T>(u, e_p, b_p, dt_2);
   px[p] = u[0];
   py[p] = u[1];
   pz[p] = u[2];
@@ -721,8 +739,8 @@ __device__ __forceinline__ void gather_sum(LD ld, const T (&C)[2][2], const T *g
     for (int i = 0; i < 2; i++)
 #pragma unroll
       for (int j = 0; j < 2; j++) {
-        e_p[k] += C[i][j] * ld(k * NQ, i, j);
-        b_p[k] += C[i][j] * ld(3 * NQ + k * NQ, i, j);
+        e_p[k] = madd(C[i][j], ld(k * NQ, i, j), e_p[k]);
+        b_p[k] = madd(C[i][j], ld(3 * NQ + k * NQ, i, j), b_p[k]);
 #pragma unroll
         for (int m = 1; m <= M; m++) {
           T er = ld(k * NQ + 2 * m - 1, i, j), ei = ld(k * NQ + 2 * m, i, j);
@@ -733,10 +751,10 @@ __device__ __forceinline__ void gather_sum(LD ld, const T (&C)[2][2], const T *g
             br = two * br;
             bi = two * bi;
           }
-          e_p[k] += C[i][j] * er * gr[m];
-          e_p[k] -= C[i][j] * ei * gi[m];
-          b_p[k] += C[i][j] * br * gr[m];
-          b_p[k] -= C[i][j] * bi * gi[m];
+          e_p[k] = madd(C[i][j] * er, gr[m], e_p[k]);
+          e_p[k] = madd(-(C[i][j] * ei), gi[m], e_p[k]);
+          b_p[k] = madd(C[i][j] * br, gr[m], b_p[k]);
+          b_p[k] = madd(-(C[i][j] * bi), gi[m], b_p[k]);
         }
       }
 }
@@ -849,25 +867,7 @@ __global__ void __launch_bounds__(GS_NT)
       return field_val<T, M>(fp, v, igrid + j + (int64_t)Nx * i);
     }, C, gr, gi, e_p, b_p);
   }
-  // Boris rotation, grid_deposit_m1.cl:472-507
-  T um[3], up[3], u0[3], t[3], s[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) um[k] = u[k] + dt_2 * e_p[k];
-  T g_p_inv = (T)1. / dsqrt<T>((T)1. + um[0] * um[0] + um[1] * um[1] + um[2] * um[2]);
-#pragma unroll
-  for (int k = 0; k < 3; k++) t[k] = dt_2 * b_p[k] * g_p_inv;
-  T t2 = (T)2. / ((T)1. + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-#pragma unroll
-  for (int k = 0; k < 3; k++) s[k] = t[k] * t2;
-  u0[0] = um[0] + um[1] * t[2] - um[2] * t[1];
-  u0[1] = um[1] - um[0] * t[2] + um[2] * t[0];
-  u0[2] = um[2] + um[0] * t[1] - um[1] * t[0];
-  up[0] = um[0] + u0[1] * s[2] - u0[2] * s[1];
-  up[1] = um[1] - u0[0] * s[2] + u0[2] * s[0];
-  up[2] = um[2] + u0[0] * s[1] - u0[1] * s[0];
-#pragma unroll
-  for (int k = 0; k < 3; k++) u[k] = up[k] + dt_2 * e_p[k];
-  g_p_inv = (T)1. / dsqrt<T>((T)1. + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  const T g_p_inv = boris<T>(u, e_p, b_p, dt_2);
   px[p] = u[0];
   py[p] = u[1];
   pz[p] = u[2];
