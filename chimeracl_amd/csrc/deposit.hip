@@ -635,11 +635,11 @@ __global__ void __launch_bounds__(256)
     k_gather_push(const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
                   T *__restrict__ px, T *__restrict__ py, T *__restrict__ pz,
                   T *__restrict__ g_inv, int64_t Np, const uint32_t *__restrict__ np_stay,
-                  T factor_push, int Nx, int Nr, T xmin, T dx_inv, T rmin, T dr_inv,
-                  FieldPtrs fp) {
+                  const uint32_t *__restrict__ pid, T factor_push, int Nx, int Nr, T xmin,
+                  T dx_inv, T rmin, T dr_inv, FieldPtrs fp) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= Np) return;
-  if (!(p < (int64_t)(*np_stay))) return;
+  if (!((pid ? (int64_t)pid[p] : p) < (int64_t)(*np_stay))) return;
   const int Nx_cell = Nx - 1, Nr_cell = Nr - 1;
   T xp = x[p], yp = y[p], zp = z[p];
   T rp = dsqrt<T>(yp * yp + zp * zp);
@@ -787,8 +787,9 @@ __global__ void __launch_bounds__(GS_NT)
     k_gather_sorted(const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
                     T *__restrict__ px, T *__restrict__ py, T *__restrict__ pz,
                     T *__restrict__ g_inv, const uint32_t *__restrict__ sort_indx,
-                    const uint32_t *__restrict__ np_stay_p, T factor_push, int Nx, int Nr, T xmin,
-                    T dx_inv, T rmin, T dr_inv, FieldPtrs fp) {
+                    const uint32_t *__restrict__ np_stay_p, const uint32_t *__restrict__ pid,
+                    T factor_push, int Nx, int Nr, T xmin, T dx_inv, T rmin, T dr_inv,
+                    FieldPtrs fp) {
   typedef GatherCfg<T, M> Cfg;
   constexpr int NV = Cfg::NV, MAXN = Cfg::MAXN;
   __shared__ T fs[2 * MAXN * NV];
@@ -835,7 +836,7 @@ __global__ void __launch_bounds__(GS_NT)
     stage_node<T, M>(fp, node, fs + threadIdx.x * NV);
   }
   __syncthreads();
-  if (!live || !((int64_t)p < np_stay)) return;
+  if (!live || !((int64_t)(pid ? pid[p] : p) < np_stay)) return;
   const int64_t igrid = ix + (int64_t)ir * Nx;
   const int lc = ix - ixA;
   T dt_2 = (T)0.5 * factor_push;
@@ -877,8 +878,8 @@ __global__ void __launch_bounds__(GS_NT)
 template <typename T>
 static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, T *g_inv,
                        int64_t Np, const uint32_t *sort_indx, const uint32_t *np_stay,
-                       T factor_push, const chm_grid_t *g, const chm_planes_t *E,
-                       const chm_planes_t *B, void *stream) {
+                       const uint32_t *pid, T factor_push, const chm_grid_t *g,
+                       const chm_planes_t *E, const chm_planes_t *B, void *stream) {
   if (!g || !E || !B || g->M < 0 || g->M > CHM_MAXM || Np < 0) return CHM_EINVAL;
   if (Np == 0) return CHM_OK;
   FieldPtrs fp = {};
@@ -897,10 +898,10 @@ static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, 
   case MM:                                                                                         \
     if (sort_indx)                                                                                 \
       k_gather_sorted<T, MM><<<chm_blocks(Np, GS_NT), GS_NT, 0, st>>>(                             \
-          x, y, z, px, py, pz, g_inv, sort_indx, np_stay, factor_push, g->Nx, g->Nr, (T)g->xmin,   \
-          (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp);                                             \
+          x, y, z, px, py, pz, g_inv, sort_indx, np_stay, pid, factor_push, g->Nx, g->Nr,          \
+          (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp);                                 \
     else                                                                                           \
-      k_gather_push<T, MM><<<nb, 256, 0, st>>>(x, y, z, px, py, pz, g_inv, Np, np_stay,            \
+      k_gather_push<T, MM><<<nb, 256, 0, st>>>(x, y, z, px, py, pz, g_inv, Np, np_stay, pid,       \
                                                factor_push, g->Nx, g->Nr, (T)g->xmin,              \
                                                (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp);        \
     break;
@@ -916,14 +917,33 @@ extern "C" int chm_gather_push_f64(const double *x, const double *y, const doubl
                                    const uint32_t *sort_indx, const uint32_t *np_stay,
                                    double factor_push, const chm_grid_t *g, const chm_planes_t *E,
                                    const chm_planes_t *B, void *stream) {
-  return gather_push<double>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, factor_push, g, E,
-                             B, stream);
+  return gather_push<double>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, nullptr,
+                             factor_push, g, E, B, stream);
 }
 extern "C" int chm_gather_push_f32(const float *x, const float *y, const float *z, float *px,
                                    float *py, float *pz, float *g_inv, int64_t Np,
                                    const uint32_t *sort_indx, const uint32_t *np_stay,
                                    float factor_push, const chm_grid_t *g, const chm_planes_t *E,
                                    const chm_planes_t *B, void *stream) {
-  return gather_push<float>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, factor_push, g, E,
-                            B, stream);
+  return gather_push<float>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, nullptr,
+                            factor_push, g, E, B, stream);
+}
+extern "C" int chm_gather_push_ids_f64(const double *x, const double *y, const double *z,
+                                       double *px, double *py, double *pz, double *g_inv,
+                                       int64_t Np, const uint32_t *sort_indx,
+                                       const uint32_t *np_stay, const uint32_t *pid,
+                                       double factor_push, const chm_grid_t *g,
+                                       const chm_planes_t *E, const chm_planes_t *B,
+                                       void *stream) {
+  return gather_push<double>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, pid, factor_push,
+                             g, E, B, stream);
+}
+extern "C" int chm_gather_push_ids_f32(const float *x, const float *y, const float *z, float *px,
+                                       float *py, float *pz, float *g_inv, int64_t Np,
+                                       const uint32_t *sort_indx, const uint32_t *np_stay,
+                                       const uint32_t *pid, float factor_push,
+                                       const chm_grid_t *g, const chm_planes_t *E,
+                                       const chm_planes_t *B, void *stream) {
+  return gather_push<float>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, pid, factor_push,
+                            g, E, B, stream);
 }

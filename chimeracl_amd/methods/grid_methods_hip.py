@@ -152,7 +152,10 @@ class GridMethodsHIP(GenericMethodsHIP):
         comps = self.Args["vec_comps"]
         E = self._planes_arr([flds[0] + c for c in comps])
         B = self._planes_arr([flds[1] + c for c in comps])
-        self.call("chm_gather_push", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]),
-                  ptr(d["py"]), ptr(d["pz"]), ptr(d["g_inv"]), parts.Args["Np"],
-                  ptr(d["sort_indx"]), ptr(d["Np_stay"]), parts.Args["FactorPush"],
-                  self._grid(), E, B)
+        args = (ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]), ptr(d["py"]), ptr(d["pz"]),
+                ptr(d["g_inv"]), parts.Args["Np"], ptr(d["sort_indx"]), ptr(d["Np_stay"]))
+        if "pid" in d:  # storage reordered: the skip uses the unreordered index
+            self.call("chm_gather_push_ids", *args, ptr(d["pid"]), parts.Args["FactorPush"],
+                      self._grid(), E, B)
+        else:
+            self.call("chm_gather_push", *args, parts.Args["FactorPush"], self._grid(), E, B)

@@ -47,6 +47,11 @@ class ParticleMethodsHIP(GenericMethodsHIP):
             old = self.DataDev.get(arg)
             self.DataDev[arg] = new.contiguous().clone() if (old is None or old.numel() == 0) \
                 else torch.cat([old, new])
+        pid = self.DataDev.get("pid")
+        if pid is not None:  # appended particles take the next indices
+            n = self.DataDev["x"].numel()
+            self.DataDev["pid"] = torch.cat([pid, torch.arange(pid.numel(), n, dtype=torch.int32,
+                                                               device=pid.device)])
         self.reset_num_parts()
         self.flag_sorted = False
         self.version += 1
@@ -212,9 +217,17 @@ class ParticleMethodsHIP(GenericMethodsHIP):
             dst = torch.empty(Np_stay, dtype=src.dtype, device=src.device)
             self.call_raw("chm_align_elems", ptr(srt), Np_stay, ptr(src), ptr(dst), src.element_size())
             self.DataDev[comp] = dst
+        self.DataDev.pop("pid", None)  # the reference renumbers the particles here
         self.reset_num_parts()
         self.flag_sorted = False
         self.version += 1
+
+    def ensure_pid(self):
+        """DataDev['pid'] (the index of each particle without reordering),
+        the identity if the storage has not been reordered yet."""
+        d = self.DataDev
+        if "pid" not in d and self.Args["Np"] > 0:
+            d["pid"] = torch.arange(self.Args["Np"], dtype=torch.int32, device=d["x"].device)
 
     def reorder_parts(self):
         """Storage order only, no counterpart in the reference: every
@@ -222,18 +235,22 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         order with the align kernel (none dropped), and sort_indx becomes the
         identity, so the particles' memory order follows their cells again.
         Values, cell offsets and Np_stay are unchanged; particle indices
-        change.  Needs a current sort (flag_sorted)."""
+        change, and DataDev['pid'] keeps each particle's index in the
+        numbering without reordering (the gather's skip of particles with
+        index >= Np_stay uses it, so the same particles are skipped).  Needs
+        a current sort (flag_sorted)."""
         Np = self.Args["Np"]
         if Np == 0 or not self.flag_sorted:
             return
         d = self.DataDev
+        self.ensure_pid()
         if self._align_buf is None or self._align_buf[0].numel() != Np:
             dev = d["x"].device
             self._align_buf = (torch.empty(Np, dtype=torch.float64, device=dev),
                                torch.arange(Np, dtype=torch.int32, device=dev))
         scr, iota = self._align_buf
         srt = d["sort_indx"]
-        for comp in self._arg_names():
+        for comp in self._arg_names() + ["pid"]:
             a = d[comp]
             tmp = scr.view(torch.uint8)[: a.numel() * a.element_size()].view(a.dtype)
             self.call_raw("chm_align_elems", ptr(srt), Np, ptr(a), ptr(tmp), a.element_size())

@@ -125,10 +125,15 @@ class PIC_loop:
     def step(self):
         for diag in self.diags:
             diag.make_record(self.it)
-        if self.align_every and self.it > 0 and self.it % self.align_every == 0:
+        if self.align_every:
             for parts in self.species:
                 if "Immobile" not in parts.Args:
-                    parts.reorder_parts()
+                    # the gather reads pid from the first step on, so that a
+                    # captured step does not change when the first reorder
+                    # creates it
+                    parts.ensure_pid()
+                    if self.it > 0 and self.it % self.align_every == 0:
+                        parts.reorder_parts()
         if self.graph and self._graph is None and self.it >= 1 and not self.frames and not self.timit:
             self._capture()
         if self._graph is not None:

@@ -311,6 +311,25 @@ int chm_gather_push_f32(const float *x, const float *y, const float *z,
                         const uint32_t *np_stay, float factor_push,
                         const chm_grid_t *g, const chm_planes_t *E,
                         const chm_planes_t *B, void *stream);
+/* The same, for storage reordered by chm_align_elems (PIC_loop(align_every)):
+ * pid[p] is particle p's index in the numbering without reordering, and the
+ * reference's skip (grid_deposit_m1.cl:367, sorting_indx[ip] < Np_stay)
+ * compares pid[p] with *np_stay instead of p, so the same particles are
+ * skipped as without the reordering. */
+int chm_gather_push_ids_f64(const double *x, const double *y, const double *z,
+                            double *px, double *py, double *pz, double *g_inv,
+                            int64_t Np, const uint32_t *sort_indx,
+                            const uint32_t *np_stay, const uint32_t *pid,
+                            double factor_push, const chm_grid_t *g,
+                            const chm_planes_t *E, const chm_planes_t *B,
+                            void *stream);
+int chm_gather_push_ids_f32(const float *x, const float *y, const float *z,
+                            float *px, float *py, float *pz, float *g_inv,
+                            int64_t Np, const uint32_t *sort_indx,
+                            const uint32_t *np_stay, const uint32_t *pid,
+                            float factor_push, const chm_grid_t *g,
+                            const chm_planes_t *E, const chm_planes_t *B,
+                            void *stream);
 
 /* ------------------------------------------------------------- transforms */
 size_t chm_transform_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M,

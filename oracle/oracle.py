@@ -77,7 +77,7 @@ def lib():
                                          i32, f64, f64, i32, P, P]
         L.orc_deposit_vector.argtypes = [P, P, P, P, P, P, P, P, P, P, i32, i32,
                                          f64, f64, i32, f64, f64, i32, P, P]
-        L.orc_gather_push.argtypes = [P, P, P, P, P, P, P, P, i64, i64, f64, i32,
+        L.orc_gather_push.argtypes = [P, P, P, P, P, P, P, P, P, i64, i64, f64, i32,
                                       f64, f64, i32, f64, f64, i32, P, P, P, P]
         L.orc_profile_by_interpolant.argtypes = [P, P, i64, P, P, P, i32]
         L.orc_num_threads.restype = ctypes.c_int
@@ -282,6 +282,10 @@ class OracleParticles:
             new = np.ascontiguousarray(arrays[k], dtype=np.float64)
             old = self.data.get(k)
             self.data[k] = new.copy() if old is None else np.concatenate([old, new])
+        if "pid" in self.data:  # appended particles take the next indices
+            n0 = self.data["pid"].size
+            self.data["pid"] = np.concatenate(
+                [self.data["pid"], np.arange(n0, self.data["x"].size, dtype=np.uint32)])
         self.Args["Np"] = self.data["x"].size
         self.Args["Np_stay"] = self.Args["Np"]
         self.flag_sorted = False
@@ -389,6 +393,7 @@ class OracleParticles:
         idx = self.data["sort_indx"][: self.Args["Np_stay"]].astype(np.int64)
         for k in self._keys():
             self.data[k] = np.ascontiguousarray(self.data[k][idx])
+        self.data.pop("pid", None)  # the reference renumbers the particles here
         self.Args["Np"] = self.Args["Np_stay"] = idx.size
         self.flag_sorted = False
 
@@ -396,11 +401,15 @@ class OracleParticles:
         """The storage reorder of the product (not in the reference): every
         component of all Np particles permuted by sort_indx with the
         data_align gather (particles_generic.cl:156-186), none dropped;
-        sort_indx becomes the identity, cell offsets and Np_stay stay."""
+        sort_indx becomes the identity, cell offsets and Np_stay stay.  pid
+        keeps each particle's index in the unreordered numbering (for the
+        gather's K5 skip)."""
         if self.Args["Np"] == 0 or not self.flag_sorted:
             return
         idx = self.data["sort_indx"].astype(np.int64)
-        for k in self._keys():
+        if "pid" not in self.data:
+            self.data["pid"] = np.arange(self.Args["Np"], dtype=np.uint32)
+        for k in self._keys() + ["pid"]:
             self.data[k] = np.ascontiguousarray(self.data[k][idx])
         self.data["sort_indx"] = np.arange(self.Args["Np"], dtype=np.uint32)
 
@@ -711,7 +720,8 @@ class OracleSolver:
             d = parts.data
             lib().orc_gather_push(_p(d["x"]), _p(d["y"]), _p(d["z"]), _p(d["px"]),
                                   _p(d["py"]), _p(d["pz"]), _p(d["g_inv"]),
-                                  _p(d["sort_indx"]), parts.Args["Np"],
+                                  _p(d["sort_indx"]), _p(d["pid"]) if "pid" in d else None,
+                                  parts.Args["Np"],
                                   parts.Args["Np_stay"], parts.Args["FactorPush"],
                                   a["Nx"], a["Xmin"], a["dx_inv"], a["Nr"], a["Rmin"],
                                   a["dr_inv"], M, _p(E0), _p(B0), _p(Ec), _p(Bc))

@@ -261,13 +261,15 @@ void orc_deposit_vector(const uint32_t *sort_indx, const uint32_t *cell_offset,
  * gather_and_push, grid_deposit_m1.cl:330-511 (m0: grid_deposit_m0.cl:280-427),
  * launched per sorted slot as in grid_methods_cl.py:168-192.  The result of
  * each particle does not depend on the slot order, so this loops over the
- * slots exactly as the reference does.  factor_push is Particles
+ * slots exactly as the reference does.  pid (may be NULL) holds each
+ * particle's index in the unreordered numbering.  factor_push is Particles
  * Args['FactorPush'] = 2*pi*dt*q/m (particles.py:90); the kernel halves it.
  * E_m0/B_m0 [3][Nr][Nx]; E_mc/B_mc [3][M][Nr][Nx][2].
  */
 void orc_gather_push(const double *x, const double *y, const double *z,
                      double *px, double *py, double *pz, double *g_inv,
-                     const uint32_t *sort_indx, int64_t Np, int64_t Np_stay,
+                     const uint32_t *sort_indx, const uint32_t *pid, int64_t Np,
+                     int64_t Np_stay,
                      double factor_push, int32_t Nx, double xmin,
                      double dx_inv, int32_t Nr, double rmin, double dr_inv,
                      int32_t M, const double *E_m0, const double *B_m0,
@@ -277,7 +279,9 @@ void orc_gather_push(const double *x, const double *y, const double *z,
 #pragma omp parallel for schedule(static)
   for (int64_t ip = 0; ip < Np; ip++) {
     uint32_t p = sort_indx[ip];
-    if (!((int64_t)p < Np_stay)) continue;
+    /* K5: the skip compares the particle's index without reordering
+     * (pid, when the storage has been reordered) with Np_stay */
+    if (!((int64_t)(pid ? pid[p] : p) < Np_stay)) continue;
     double xp = x[p], yp = y[p], zp = z[p];
     double rp = sqrt(yp * yp + zp * zp);
     int ix = (int)floor((xp - xmin) * dx_inv);
