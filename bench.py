@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C2]
                     [--no-cpu] [--no-mixed] [--no-diff] [--align-every K]
-                    [--aligned-section K]
+                    [--compare-align K]
 
 Workload (default): the medium laser-wakefield configuration, Nz=2048,
 Nr=256, modes m=0,1, 2x2x8 particles per cell (16.6M electrons plus as many
@@ -34,11 +34,16 @@ rerun eagerly with HIP events around the gather in each step; the rerun must
 end bit-identical to the timed run.  traffic: HBM bytes per launch from the
 rocprofv3 counter summary in profiles/.
 
-value follows the reference's loop exactly (particle storage never
-reordered).  The "aligned" section times both forms again with the electrons
-permuted into their sorted order every K steps (PIC_loop(align_every=K), an
-addition of this package): the same physics with another particle numbering,
-and the memory order of the particles kept close to their cells.
+Particle storage: every 20 steps the electrons are permuted into their
+sorted order (PIC_loop(align_every=20), the reference's align gather over all
+particles), so that their memory order keeps following their cells, as the
+reference's own moving-window runs realign at every frame step.  Each
+electron keeps its unreordered index for the gather's Np_stay skip, so the
+run is the reference's step with the same particles skipped; only the
+summation order within a cell follows the storage (the reference's in-cell
+order is nondeterministic).  The "align_compare" section times both forms
+again without any reordering (align_every=0, the reference loop's storage
+order) over the same steps.
 
 Printed: one JSON line.
 """
@@ -73,10 +78,10 @@ def parse():
     ap.add_argument("--cpu-warmup", type=int, default=2)
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--diag-steps", default="100,500")
-    ap.add_argument("--align-every", type=int, default=0,
-                    help="permute the electrons into their sorted order every K steps")
-    ap.add_argument("--aligned-section", type=int, default=20,
-                    help="also time both forms with align_every=K (0: skip)")
+    ap.add_argument("--align-every", type=int, default=20,
+                    help="permute the electrons into their sorted order every K steps (0: never)")
+    ap.add_argument("--compare-align", type=int, default=0,
+                    help="also time both forms with align_every=K (-1: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (HBM counter calibration)")
     ap.add_argument("--no-configs", action="store_true",
@@ -105,7 +110,7 @@ def gather_only(args):
     import torch
     out = {}
     for prec in ("f64", "mixed"):
-        sm = Sim(args.config, prec, False)
+        sm = Sim(args.config, prec, False, args.align_every)
         sm.loop.graph = False
         sm.step(args.gather_only)
         for _ in range(3):
@@ -200,7 +205,10 @@ def oracle_errors(sim, M):
         out[f] = (num / den) ** 0.5 if den > 0 else num ** 0.5
     num = den = 0.0
     for k in ("px", "py", "pz"):
-        a, b = sim.eons.host(k), e.data[k]
+        a = by_pid(sim.eons, k).double().cpu().numpy()
+        b = e.data[k] if "pid" not in e.data else np.empty_like(e.data[k])
+        if "pid" in e.data:
+            b[e.data["pid"].astype(np.int64)] = e.data[k]
         num += float(np.linalg.norm(a - b) ** 2)
         den += float(np.linalg.norm(b) ** 2)
     out["p"] = (num / den) ** 0.5
@@ -230,7 +238,20 @@ def mixed_errors(mx, fp, M):
                 b = fp.solver.DataDev[f"{f}{c}_m{m}"][1:]
                 pairs.append((a.to(torch.complex128 if b.is_complex() else torch.float64), b))
         out[f] = _rel(pairs)
-    out["p"] = _rel([(mx.eons.DataDev[k].double(), fp.eons.DataDev[k]) for k in ("px", "py", "pz")])
+    out["p"] = _rel([(by_pid(mx.eons, k).double(), by_pid(fp.eons, k)) for k in ("px", "py", "pz")])
+    return out
+
+
+def by_pid(parts, k):
+    """Particle component k in the numbering without reordering (the two
+    forms reorder their storage independently)."""
+    import torch
+    a = parts.DataDev[k]
+    pid = parts.DataDev.get("pid")
+    if pid is None:
+        return a
+    out = torch.empty_like(a)
+    out[pid.long()] = a
     return out
 
 
@@ -428,6 +449,7 @@ def replay_timed_gather(sim, steps):
     try:
         for i in range(steps):
             cur[0] = i
+            sim.loop.pre_step()
             sim.loop._body()
             sim.loop.it += 1
     finally:
@@ -595,9 +617,9 @@ def diff_section():
 
 
 def aligned_section(args, k):
-    """Both forms again, with the electrons permuted into their sorted order
-    every k steps (PIC_loop(align_every=k)): same workload, warmup and timed
-    window as the main measurement."""
+    """Both forms again with PIC_loop(align_every=k) (k = 0: never
+    reordered): same workload, warmup and timed window as the main
+    measurement."""
     import torch
     out = {"align_every": k}
     first = min([int(v) for v in args.diag_steps.split(",") if v] or [0])
@@ -623,7 +645,7 @@ def aligned_section(args, k):
     return out
 
 
-def config_section(name, diag_at, oracle_steps=0):
+def config_section(name, diag_at, oracle_steps=0, align_every=0):
     """Configuration `name` in both forms, in lockstep: at the steps in
     diag_at, the mixed run's relative L2 errors against the FP64 run (E, B,
     J, momenta), both charge residuals and energies; with oracle_steps, the
@@ -631,14 +653,14 @@ def config_section(name, diag_at, oracle_steps=0):
     import torch
     from chimeracl_amd import config as C
     t_all = time.time()
-    fp = Sim(name, "f64", oracle_steps > 0)
-    mx = Sim(name, "mixed", False)
+    fp = Sim(name, "f64", oracle_steps > 0, align_every)
+    mx = Sim(name, "mixed", False, align_every)
     M = fp.solver.Args["M"]
     add_div_scratch(fp.solver)
     cfg = C.CONFIGS[name]
     out = {"workload": f"{name}: " + cfg["desc"], "Nz": cfg["Nx"], "Nr": cfg["Nr"], "modes": M + 1,
            "ppc": list(cfg["ppc"]), "electrons": fp.eons.Args["Np"], "steps": max(diag_at),
-           "profile": cfg.get("profile")}
+           "profile": cfg.get("profile"), "align_every": align_every}
     targets = sorted(set(diag_at) | ({oracle_steps} if oracle_steps else set()))
     t_gpu = 0.0
     for t in targets:
@@ -826,11 +848,12 @@ def main():
     del fp, mx, sims
     gc.collect()
     torch.cuda.empty_cache()
-    aligned = aligned_section(args, args.aligned_section) if args.aligned_section > 0 else None
+    aligned = (aligned_section(args, args.compare_align)
+               if args.compare_align >= 0 and args.compare_align != args.align_every else None)
     configs = None
     if not args.no_configs and args.config == "C2":
-        configs = {"C1": config_section("C1", [100, 200], oracle_steps=0 if args.no_cpu else 5),
-                   "C3": config_section("C3", [100, 1000])}
+        configs = {"C1": config_section("C1", [100, 200], 0 if args.no_cpu else 5, args.align_every),
+                   "C3": config_section("C3", [100, 1000], 0, args.align_every)}
     diff = None
     if not args.no_diff:
         t0 = time.time()
@@ -861,7 +884,7 @@ def main():
         "diagnostics": diags,
         "kernels": kernels,
         "moving_window": window,
-        "aligned": aligned,
+        "align_compare": aligned,
         "configs": configs,
         "differentiable": diff,
     }

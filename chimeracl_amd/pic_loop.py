@@ -123,17 +123,7 @@ class PIC_loop:
             self.timer_record("gather + push-p")
 
     def step(self):
-        for diag in self.diags:
-            diag.make_record(self.it)
-        if self.align_every:
-            for parts in self.species:
-                if "Immobile" not in parts.Args:
-                    # the gather reads pid from the first step on, so that a
-                    # captured step does not change when the first reorder
-                    # creates it
-                    parts.ensure_pid()
-                    if self.it > 0 and self.it % self.align_every == 0:
-                        parts.reorder_parts()
+        self.pre_step()
         if self.graph and self._graph is None and self.it >= 1 and not self.frames and not self.timit:
             self._capture()
         if self._graph is not None:
@@ -146,6 +136,21 @@ class PIC_loop:
             self._flush_timers()
         self.it += 1
         return self.it
+
+    def pre_step(self):
+        """What a step does before its kernels: diagnostics records, then
+        (align_every) the reorder of the mobile species."""
+        for diag in self.diags:
+            diag.make_record(self.it)
+        if self.align_every:
+            for parts in self.species:
+                if "Immobile" not in parts.Args:
+                    # the gather reads pid from the first step on, so that a
+                    # captured step does not change when the first reorder
+                    # creates it
+                    parts.ensure_pid()
+                    if self.it > 0 and self.it % self.align_every == 0:
+                        parts.reorder_parts()
 
     def _capture(self):
         # one eager step has run, so every buffer exists; ions stay sorted
