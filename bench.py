@@ -106,15 +106,16 @@ def calibrate_copy():
 
 def gather_only(args):
     """Both forms of the workload, `args.gather_only` eager PIC steps each
-    (no HIP graph), then three gather + push launches per form."""
+    (no HIP graph), then three gather + push launches per form (with the
+    next first half push, as in the timed steps)."""
     import torch
     out = {}
     for prec in ("f64", "mixed"):
         sm = Sim(args.config, prec, False, args.align_every)
         sm.loop.graph = False
         sm.step(args.gather_only)
-        for _ in range(3):
-            sm.solver._gather_and_push(sm.eons, ["E", "B"])
+        for _ in range(3):  # as in the timed steps, with the next first half push
+            sm.solver._gather_and_push(sm.eons, ["E", "B"], prepush=True)
         torch.cuda.synchronize()
         out[prec] = {"steps": args.gather_only, "gather_launches": 3, "electrons": sm.eons.Args["Np"]}
         del sm
@@ -186,8 +187,8 @@ class Sim:
             self.orc = (s, e, i, O.OracleLoop(s, [e, i], align_every=align_every))
 
     def step(self, n=1):
-        for _ in range(n):
-            self.loop.step()
+        if n > 0:
+            self.loop.run(n)
         self.steps += n
 
 
@@ -426,11 +427,11 @@ def fingerprint(sim):
 
 
 def replay_timed_gather(sim, steps):
-    """Runs `steps` PIC steps eagerly (the kernels the graph replays, in the
-    same order) with a pair of HIP events around the electrons' gather + push
-    in every step, on the stream it is launched on; returns the mean, min and
-    max of its duration (ms).  The host enqueues ahead of the GPU, so each
-    pair brackets the kernel alone."""
+    """Runs `steps` PIC steps eagerly (PIC_loop.run without graphs: the
+    kernels the graphs replay, in the same order) with a pair of HIP events
+    around the electrons' gather + push in every step, on the stream it is
+    launched on; returns the mean, min and max of its duration (ms).  The
+    host enqueues ahead of the GPU, so each pair brackets the kernel alone."""
     import torch
     s, e = sim.solver, sim.eons
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -438,33 +439,39 @@ def replay_timed_gather(sim, steps):
     cur = [0]
     orig = s._gather_and_push
 
-    def timed(parts, flds):
+    pre = [False] * steps
+
+    def timed(parts, flds, **kw):
         if parts is e:
             ev[cur[0]][0].record()
-        orig(parts, flds)
+            pre[cur[0]] = bool(kw.get("prepush"))
+        orig(parts, flds, **kw)
         if parts is e:
             ev[cur[0]][1].record()
+            cur[0] += 1
 
     s._gather_and_push = timed
+    graph, sim.loop.graph = sim.loop.graph, False
     try:
-        for i in range(steps):
-            cur[0] = i
-            sim.loop.pre_step()
-            sim.loop._body()
-            sim.loop.it += 1
+        sim.loop.run(steps)  # eager, the same step sequence as the timed run
     finally:
         del s._gather_and_push
+        sim.loop.graph = graph
     torch.cuda.synchronize()
     ms = [a.elapsed_time(b) for a, b in ev]
     return {"mean_ms": float(np.mean(ms)), "min_ms": float(np.min(ms)), "max_ms": float(np.max(ms)),
-            "steps": steps}
+            "steps": steps, "prepush_steps": int(sum(pre))}
 
 
 def window_roofline(sim, gm, w):
     """Roofline of the gather + push over the timed steps: algorithmic bytes
-    (10 words per electron) over its mean duration in those steps."""
+    over its mean duration in those steps.  Per electron: x, y, z, px, py, pz
+    read and px, py, pz, g_inv written (10 words); in a step whose gather also
+    applies the next first half push, the pushed x, y, z and the key written
+    as well (3 words + 4 bytes)."""
     t = gm["mean_ms"] * 1e-3
-    nbytes = sim.eons.Args["Np"] * 10 * w
+    Np = sim.eons.Args["Np"]
+    nbytes = Np * 10 * w + Np * (3 * w + 4) * gm["prepush_steps"] / gm["steps"]
     achieved = nbytes / t / 1e9
     return {"kernel": "k_gather_push", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -630,8 +637,7 @@ def aligned_section(args, k):
         sm.step(max(args.warmup, first))
         torch.cuda.synchronize()
         t0 = time.time()
-        for _ in range(args.steps):
-            sm.loop.step()
+        sm.loop.run(args.steps)
         torch.cuda.synchronize()
         dt = time.time() - t0
         Ne = sm.eons.Args["Np"]
@@ -800,8 +806,7 @@ def main():
         it0 = sm.loop.it
         torch.cuda.synchronize()
         t0 = time.time()
-        for _ in range(args.steps):
-            sm.loop.step()
+        sm.loop.run(args.steps)
         torch.cuda.synchronize()
         dt = time.time() - t0
         timing[name] = dt

@@ -73,10 +73,10 @@ _SIGS = {
     "chm_deposit_scalar_f32": [P, P, P, P, P, P, i32, R(Grid), R(Planes), P, sz, P],
     "chm_deposit_vector_f64": [P, P, P, P, P, P, P, P, P, P, i32, R(Grid), R(Planes), P, sz, P],
     "chm_deposit_vector_f32": [P, P, P, P, P, P, P, P, P, P, i32, R(Grid), R(Planes), P, sz, P],
-    "chm_deposit_vector_push_f64": [P, P, P, P, P, P, P, P, P, P, i32, f64, i64, P, R(Grid),
-                                    R(Planes), P, sz, P],
-    "chm_deposit_vector_push_f32": [P, P, P, P, P, P, P, P, P, P, i32, f32, i64, P, R(Grid),
-                                    R(Planes), P, sz, P],
+    "chm_deposit_vector_push_f64": [P, P, P, P, P, P, P, P, P, P, P, P, P, i32, f64, i64, P,
+                                    R(Grid), R(Planes), P, sz, P],
+    "chm_deposit_vector_push_f32": [P, P, P, P, P, P, P, P, P, P, P, P, P, i32, f32, i64, P,
+                                    R(Grid), R(Planes), P, sz, P],
     "chm_deposit_add_staged_f64": [P, R(Grid), R(Planes), i32, P, sz, P],
     "chm_deposit_add_staged_f32": [P, R(Grid), R(Planes), i32, P, sz, P],
     "chm_postproc_deposit_f64": [R(Planes), i32, P, i32, i32, i32, P],
@@ -89,6 +89,10 @@ _SIGS = {
                                 R(Planes), P],
     "chm_gather_push_ids_f32": [P, P, P, P, P, P, P, i64, P, P, P, f32, R(Grid), R(Planes),
                                 R(Planes), P],
+    "chm_gather_push_pre_f64": [P, P, P, P, P, P, P, i64, P, P, P, P, f64, f64, P, P, P, P,
+                                R(Grid), R(Planes), R(Planes), P],
+    "chm_gather_push_pre_f32": [P, P, P, P, P, P, P, i64, P, P, P, P, f32, f32, P, P, P, P,
+                                R(Grid), R(Planes), R(Planes), P],
     "chm_transform_workspace_bytes": ([i32, i32, i32, i32, i32], sz),
     "chm_transform_f64": [R(Spectral), R(Planes), R(Fb), i32, i32, i32, i32, P, sz, P],
     "chm_transform_f32": [R(Spectral), R(Planes), R(Fb), i32, i32, i32, i32, P, sz, P],

@@ -56,8 +56,8 @@ struct PlanePtrs {
 // adjoints of the gather (wp = 1, values in ux/uy/uz, modes weighted 2 e_m).
 // PUSH (KIND 1): the lane that stages a particle also applies the next
 // coordinate push (x += u dt g_inv, push_x's operations) and writes the
-// pushed position and its cell key (the keys of the following sort); the
-// sums use the positions before the push.
+// pushed position (to xo, which may be x) and its cell key (the keys of the
+// following sort); the sums use the positions before the push.
 template <int M, int KIND> struct DepCfg {
   static constexpr bool VEC = KIND != 0;
   static constexpr int NK = VEC ? 3 : 1;              // components
@@ -73,11 +73,11 @@ template <int M, int KIND> struct DepCfg {
 template <typename T, int M, int KIND, bool PUSH>
 __global__ void __launch_bounds__(64)
     k_cell_sums(const uint32_t *__restrict__ sort_indx, const uint32_t *__restrict__ cell_offset,
-                T *__restrict__ x, T *__restrict__ y, T *__restrict__ z,
+                const T *x, const T *y, const T *z,
                 const T *__restrict__ ux, const T *__restrict__ uy, const T *__restrict__ uz,
                 const T *__restrict__ g_inv, const T *__restrict__ w, T charge, int Nx, int Nr,
                 T xmin, T dx_inv, T rmin, T dr_inv, T *__restrict__ part, T dt,
-                uint32_t *__restrict__ key_out) {
+                uint32_t *__restrict__ key_out, T *xo, T *yo, T *zo) {
   typedef DepCfg<M, KIND> Cfg;
   constexpr bool VEC = Cfg::VEC;
   constexpr int CAP = Cfg::CAP, NV = Cfg::NV, TPC = Cfg::TPC;
@@ -126,9 +126,9 @@ __global__ void __launch_bounds__(64)
             const T dt_g = dt * gp;
             const T xn = lx[rr] + ux[p] * dt_g, yn = ly[rr] + uy[p] * dt_g,
                     zn = lz[rr] + uz[p] * dt_g;
-            x[p] = xn;
-            y[p] = yn;
-            z[p] = zn;
+            xo[p] = xn;
+            yo[p] = yn;
+            zo[p] = zn;
             key_out[p] = cell_key<T>(xn, yn, zn, Nx - 1, Nr - 1, xmin, dx_inv, rmin, dr_inv);
           }
         } else if (KIND == 0) {
@@ -234,26 +234,28 @@ __global__ void __launch_bounds__(64)
   }
 }
 
-// The coordinate push + cell key of the particles a PUSH deposit does not
-// stage: sorted slots from Np_stay (= cell_offset[ncell], the start of the
-// out-of-domain bin) to Np.  Grid-stride over a fixed grid, as Np_stay is
-// known on the device only.
+// The coordinate push + cell key of the particles a PUSH deposit (or a
+// pre-pushing gather) does not visit: sorted slots from Np_stay (=
+// cell_offset[ncell], the start of the out-of-domain bin) to Np; reads x,
+// writes xo (may be x).  Grid-stride over a fixed grid, as Np_stay is known on
+// the device only.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_push_keys_tail(const uint32_t *__restrict__ sort_indx, const uint32_t *__restrict__ cell_offset,
-                     int64_t ncell, int64_t Np, T *__restrict__ x, T *__restrict__ y,
-                     T *__restrict__ z, const T *__restrict__ ux, const T *__restrict__ uy,
+                     int64_t ncell, int64_t Np, const T *x, const T *y, const T *z,
+                     const T *__restrict__ ux, const T *__restrict__ uy,
                      const T *__restrict__ uz, const T *__restrict__ g_inv, T dt, int Nx, int Nr,
-                     T xmin, T dx_inv, T rmin, T dr_inv, uint32_t *__restrict__ key_out) {
+                     T xmin, T dx_inv, T rmin, T dr_inv, uint32_t *__restrict__ key_out, T *xo,
+                     T *yo, T *zo) {
   const int64_t np_stay = cell_offset[ncell];
   for (int64_t slot = np_stay + (int64_t)blockIdx.x * 256 + threadIdx.x; slot < Np;
        slot += (int64_t)gridDim.x * 256) {
     const uint32_t p = sort_indx[slot];
     const T dt_g = dt * g_inv[p];
     const T xn = x[p] + ux[p] * dt_g, yn = y[p] + uy[p] * dt_g, zn = z[p] + uz[p] * dt_g;
-    x[p] = xn;
-    y[p] = yn;
-    z[p] = zn;
+    xo[p] = xn;
+    yo[p] = yn;
+    zo[p] = zn;
     key_out[p] = cell_key<T>(xn, yn, zn, Nx - 1, Nr - 1, xmin, dx_inv, rmin, dr_inv);
   }
 }
@@ -315,11 +317,23 @@ extern "C" size_t chm_deposit_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M,
   return chm_align(nq * ncell * sizeof(double));
 }
 
+template <typename T>
+static void launch_push_tail(const uint32_t *sort_indx, const uint32_t *cell_offset, int64_t ncell,
+                             int64_t Np, const T *x, const T *y, const T *z, const T *ux,
+                             const T *uy, const T *uz, const T *g_inv, T dt, const chm_grid_t *g,
+                             uint32_t *keys, T *xo, T *yo, T *zo, hipStream_t st) {
+  const int64_t nt = chm_blocks(Np, 256);
+  k_push_keys_tail<T><<<(unsigned)(nt < 1024 ? (nt < 1 ? 1 : nt) : 1024), 256, 0, st>>>(
+      sort_indx, cell_offset, ncell, Np, x, y, z, ux, uy, uz, g_inv, dt, g->Nx, g->Nr, (T)g->xmin,
+      (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, keys, xo, yo, zo);
+}
+
 // Push of a PUSH deposit: time step and key output.
 template <typename T> struct DepPush {
   T dt;
   int64_t Np;
   uint32_t *keys;  // nullptr: no push
+  T *xo, *yo, *zo; // pushed positions (may be the input positions)
 };
 
 template <typename T, int M, int KIND>
@@ -332,16 +346,15 @@ static void launch_deposit(const uint32_t *sort_indx, const uint32_t *cell_offse
   const unsigned nb = chm_blocks(ncell, DepCfg<M, KIND>::CELLS);
   if (pass1 && KIND == 1 && push.keys) {
     k_cell_sums<T, M, KIND, true><<<nb, 64, 0, st>>>(
-        sort_indx, cell_offset, (T *)x, (T *)y, (T *)z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr,
-        (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, part, push.dt, push.keys);
-    int64_t nt = chm_blocks(push.Np, 256);
-    k_push_keys_tail<T><<<(unsigned)(nt < 1024 ? (nt < 1 ? 1 : nt) : 1024), 256, 0, st>>>(
-        sort_indx, cell_offset, ncell, push.Np, (T *)x, (T *)y, (T *)z, ux, uy, uz, g_inv, push.dt,
-        g->Nx, g->Nr, (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, push.keys);
+        sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr, (T)g->xmin,
+        (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, part, push.dt, push.keys, push.xo, push.yo,
+        push.zo);
+    launch_push_tail<T>(sort_indx, cell_offset, ncell, push.Np, x, y, z, ux, uy, uz, g_inv,
+                        push.dt, g, push.keys, push.xo, push.yo, push.zo, st);
   } else if (pass1) {
     k_cell_sums<T, M, KIND, false><<<nb, 64, 0, st>>>(
-        sort_indx, cell_offset, (T *)x, (T *)y, (T *)z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr,
-        (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, part, (T)0, nullptr);
+        sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr, (T)g->xmin,
+        (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, part, (T)0, nullptr, nullptr, nullptr, nullptr);
   }
   int64_t nnode = (int64_t)(g->Nr - 1) * g->Nx;
   k_node_add<T, M, KIND ? 3 : 1><<<chm_blocks(nnode, 256), 256, 0, st>>>(cell_offset, part, g->Nx,
@@ -353,7 +366,7 @@ static int deposit(const uint32_t *sort_indx, const uint32_t *cell_offset, const
                    const T *z, const T *ux, const T *uy, const T *uz, const T *g_inv, const T *w,
                    int32_t charge, const chm_grid_t *g, const chm_planes_t *flds, void *ws,
                    size_t ws_bytes, void *stream, bool pass1 = true,
-                   DepPush<T> push = DepPush<T>{(T)0, 0, nullptr}) {
+                   DepPush<T> push = DepPush<T>{(T)0, 0, nullptr, nullptr, nullptr, nullptr}) {
   if (!g || !flds || g->M < 0 || g->M > CHM_MAXM || g->Nx < 4 || g->Nr < 3) return CHM_EINVAL;
   if (chm_deposit_workspace_bytes(g->Nx, g->Nr, g->M, KIND != 0) > ws_bytes || !ws) return CHM_EWORKSPACE;
   PlanePtrs pp = {};
@@ -414,35 +427,41 @@ extern "C" int chm_deposit_vector_f32(const uint32_t *sort_indx, const uint32_t 
 }
 
 template <typename T>
-static int deposit_vector_push(const uint32_t *sort_indx, const uint32_t *cell_offset, T *x, T *y,
-                               T *z, const T *ux, const T *uy, const T *uz, const T *g_inv,
-                               const T *w, int32_t charge, T dt, int64_t Np,
-                               uint32_t *indx_in_cell, const chm_grid_t *g, const chm_planes_t *J,
-                               void *ws, size_t ws_bytes, void *stream) {
-  if (!indx_in_cell || Np < 0 || Np > 0x7FFFFFF0ll) return CHM_EINVAL;
+static int deposit_vector_push(const uint32_t *sort_indx, const uint32_t *cell_offset,
+                               const T *xin, const T *yin, const T *zin, T *x, T *y, T *z,
+                               const T *ux, const T *uy, const T *uz, const T *g_inv, const T *w,
+                               int32_t charge, T dt, int64_t Np, uint32_t *indx_in_cell,
+                               const chm_grid_t *g, const chm_planes_t *J, void *ws,
+                               size_t ws_bytes, void *stream) {
+  if (!indx_in_cell || !xin || !yin || !zin || !x || !y || !z || Np < 0 || Np > 0x7FFFFFF0ll)
+    return CHM_EINVAL;
   if (Np == 0) return CHM_OK;
-  return deposit<T, 1>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g, J, ws,
-                       ws_bytes, stream, true, DepPush<T>{dt, Np, indx_in_cell});
+  return deposit<T, 1>(sort_indx, cell_offset, xin, yin, zin, ux, uy, uz, g_inv, w, charge, g, J,
+                       ws, ws_bytes, stream, true, DepPush<T>{dt, Np, indx_in_cell, x, y, z});
 }
 extern "C" int chm_deposit_vector_push_f64(const uint32_t *sort_indx, const uint32_t *cell_offset,
-                                           double *x, double *y, double *z, const double *ux,
-                                           const double *uy, const double *uz, const double *g_inv,
-                                           const double *w, int32_t charge, double dt, int64_t Np,
-                                           uint32_t *indx_in_cell, const chm_grid_t *g,
-                                           const chm_planes_t *J, void *ws, size_t ws_bytes,
-                                           void *stream) {
-  return deposit_vector_push<double>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge,
-                                     dt, Np, indx_in_cell, g, J, ws, ws_bytes, stream);
+                                           const double *xin, const double *yin,
+                                           const double *zin, double *x, double *y, double *z,
+                                           const double *ux, const double *uy, const double *uz,
+                                           const double *g_inv, const double *w, int32_t charge,
+                                           double dt, int64_t Np, uint32_t *indx_in_cell,
+                                           const chm_grid_t *g, const chm_planes_t *J, void *ws,
+                                           size_t ws_bytes, void *stream) {
+  return deposit_vector_push<double>(sort_indx, cell_offset, xin, yin, zin, x, y, z, ux, uy, uz,
+                                     g_inv, w, charge, dt, Np, indx_in_cell, g, J, ws, ws_bytes,
+                                     stream);
 }
 extern "C" int chm_deposit_vector_push_f32(const uint32_t *sort_indx, const uint32_t *cell_offset,
+                                           const float *xin, const float *yin, const float *zin,
                                            float *x, float *y, float *z, const float *ux,
                                            const float *uy, const float *uz, const float *g_inv,
                                            const float *w, int32_t charge, float dt, int64_t Np,
                                            uint32_t *indx_in_cell, const chm_grid_t *g,
                                            const chm_planes_t *J, void *ws, size_t ws_bytes,
                                            void *stream) {
-  return deposit_vector_push<float>(sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge,
-                                    dt, Np, indx_in_cell, g, J, ws, ws_bytes, stream);
+  return deposit_vector_push<float>(sort_indx, cell_offset, xin, yin, zin, x, y, z, ux, uy, uz,
+                                    g_inv, w, charge, dt, Np, indx_in_cell, g, J, ws, ws_bytes,
+                                    stream);
 }
 
 // Scatter of the gather's per-particle field adjoints (three components,
@@ -782,14 +801,36 @@ __device__ __forceinline__ void stage_node(const FieldPtrs &fp, int64_t node, T 
   for (int v = 0; v < NV; v++) dst[v] = (v % NQ) ? (T)2 * val[v] : val[v];
 }
 
-template <typename T, int M>
+// The next step's first coordinate push, applied by a PRE gather to each
+// particle it visits (push_x's operations, from the momenta and 1/gamma the
+// particle leaves the step with), into separate position arrays, with the
+// cell key of the pushed position.
+template <typename T> struct PrePush {
+  T dt;
+  T *x, *y, *z;
+  uint32_t *keys;
+};
+
+template <typename T>
+__device__ __forceinline__ void pre_push(const PrePush<T> &pp, uint32_t p, T xp, T yp, T zp,
+                                         const T (&u)[3], T g, int Nx, int Nr, T xmin, T dx_inv,
+                                         T rmin, T dr_inv) {
+  const T dt_g = pp.dt * g;
+  const T xn = xp + u[0] * dt_g, yn = yp + u[1] * dt_g, zn = zp + u[2] * dt_g;
+  pp.x[p] = xn;
+  pp.y[p] = yn;
+  pp.z[p] = zn;
+  pp.keys[p] = cell_key<T>(xn, yn, zn, Nx - 1, Nr - 1, xmin, dx_inv, rmin, dr_inv);
+}
+
+template <typename T, int M, bool PRE>
 __global__ void __launch_bounds__(GS_NT)
     k_gather_sorted(const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
                     T *__restrict__ px, T *__restrict__ py, T *__restrict__ pz,
                     T *__restrict__ g_inv, const uint32_t *__restrict__ sort_indx,
                     const uint32_t *__restrict__ np_stay_p, const uint32_t *__restrict__ pid,
                     T factor_push, int Nx, int Nr, T xmin, T dx_inv, T rmin, T dr_inv,
-                    FieldPtrs fp) {
+                    FieldPtrs fp, PrePush<T> pp) {
   typedef GatherCfg<T, M> Cfg;
   constexpr int NV = Cfg::NV, MAXN = Cfg::MAXN;
   __shared__ T fs[2 * MAXN * NV];
@@ -836,7 +877,12 @@ __global__ void __launch_bounds__(GS_NT)
     stage_node<T, M>(fp, node, fs + threadIdx.x * NV);
   }
   __syncthreads();
-  if (!live || !((int64_t)(pid ? pid[p] : p) < np_stay)) return;
+  if (!live) return;
+  if (!((int64_t)(pid ? pid[p] : p) < np_stay)) {
+    // skipped by the reference's Np_stay test: momenta unchanged
+    if (PRE) pre_push<T>(pp, p, xp, yp, zp, u, g_inv[p], Nx, Nr, xmin, dx_inv, rmin, dr_inv);
+    return;
+  }
   const int64_t igrid = ix + (int64_t)ir * Nx;
   const int lc = ix - ixA;
   T dt_2 = (T)0.5 * factor_push;
@@ -873,13 +919,18 @@ __global__ void __launch_bounds__(GS_NT)
   py[p] = u[1];
   pz[p] = u[2];
   g_inv[p] = g_p_inv;
+  if (PRE) pre_push<T>(pp, p, xp, yp, zp, u, g_p_inv, Nx, Nr, xmin, dx_inv, rmin, dr_inv);
 }
 
 template <typename T>
 static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, T *g_inv,
                        int64_t Np, const uint32_t *sort_indx, const uint32_t *np_stay,
                        const uint32_t *pid, T factor_push, const chm_grid_t *g,
-                       const chm_planes_t *E, const chm_planes_t *B, void *stream) {
+                       const chm_planes_t *E, const chm_planes_t *B, void *stream,
+                       const uint32_t *cell_offset = nullptr,
+                       PrePush<T> pp = PrePush<T>{(T)0, nullptr, nullptr, nullptr, nullptr}) {
+  const bool pre = pp.keys != nullptr;
+  if (pre && (!sort_indx || !cell_offset || !pp.x || !pp.y || !pp.z)) return CHM_EINVAL;
   if (!g || !E || !B || g->M < 0 || g->M > CHM_MAXM || Np < 0) return CHM_EINVAL;
   if (Np == 0) return CHM_OK;
   FieldPtrs fp = {};
@@ -896,10 +947,14 @@ static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, 
   switch (g->M) {
 #define CASE(MM)                                                                                   \
   case MM:                                                                                         \
-    if (sort_indx)                                                                                 \
-      k_gather_sorted<T, MM><<<chm_blocks(Np, GS_NT), GS_NT, 0, st>>>(                             \
+    if (pre)                                                                                       \
+      k_gather_sorted<T, MM, true><<<chm_blocks(Np, GS_NT), GS_NT, 0, st>>>(                       \
           x, y, z, px, py, pz, g_inv, sort_indx, np_stay, pid, factor_push, g->Nx, g->Nr,          \
-          (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp);                                 \
+          (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp, pp);                             \
+    else if (sort_indx)                                                                            \
+      k_gather_sorted<T, MM, false><<<chm_blocks(Np, GS_NT), GS_NT, 0, st>>>(                      \
+          x, y, z, px, py, pz, g_inv, sort_indx, np_stay, pid, factor_push, g->Nx, g->Nr,          \
+          (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp, pp);                             \
     else                                                                                           \
       k_gather_push<T, MM><<<nb, 256, 0, st>>>(x, y, z, px, py, pz, g_inv, Np, np_stay, pid,       \
                                                factor_push, g->Nx, g->Nr, (T)g->xmin,              \
@@ -907,6 +962,12 @@ static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, 
     break;
     CASE(0) CASE(1) CASE(2) CASE(3) CASE(4)
 #undef CASE
+  }
+  if (pre) {
+    // the particles of the out-of-domain bin, which the gather does not visit
+    const int64_t ncell = (int64_t)(g->Nx - 1) * (g->Nr - 1);
+    launch_push_tail<T>(sort_indx, cell_offset, ncell, Np, x, y, z, px, py, pz, g_inv, pp.dt, g,
+                        pp.keys, pp.x, pp.y, pp.z, st);
   }
   CHM_CHECK_LAUNCH();
   return CHM_OK;
@@ -946,4 +1007,31 @@ extern "C" int chm_gather_push_ids_f32(const float *x, const float *y, const flo
                                        const chm_planes_t *B, void *stream) {
   return gather_push<float>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, pid, factor_push,
                             g, E, B, stream);
+}
+extern "C" int chm_gather_push_pre_f64(const double *x, const double *y, const double *z,
+                                       double *px, double *py, double *pz, double *g_inv,
+                                       int64_t Np, const uint32_t *sort_indx,
+                                       const uint32_t *cell_offset, const uint32_t *np_stay,
+                                       const uint32_t *pid, double factor_push, double dt,
+                                       double *xh, double *yh, double *zh,
+                                       uint32_t *indx_in_cell, const chm_grid_t *g,
+                                       const chm_planes_t *E, const chm_planes_t *B,
+                                       void *stream) {
+  if (!indx_in_cell) return CHM_EINVAL;
+  return gather_push<double>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, pid, factor_push,
+                             g, E, B, stream, cell_offset,
+                             PrePush<double>{dt, xh, yh, zh, indx_in_cell});
+}
+extern "C" int chm_gather_push_pre_f32(const float *x, const float *y, const float *z, float *px,
+                                       float *py, float *pz, float *g_inv, int64_t Np,
+                                       const uint32_t *sort_indx, const uint32_t *cell_offset,
+                                       const uint32_t *np_stay, const uint32_t *pid,
+                                       float factor_push, float dt, float *xh, float *yh,
+                                       float *zh, uint32_t *indx_in_cell, const chm_grid_t *g,
+                                       const chm_planes_t *E, const chm_planes_t *B,
+                                       void *stream) {
+  if (!indx_in_cell) return CHM_EINVAL;
+  return gather_push<float>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, pid, factor_push,
+                            g, E, B, stream, cell_offset,
+                            PrePush<float>{dt, xh, yh, zh, indx_in_cell});
 }

@@ -28,10 +28,13 @@ class Grid(GridMethodsHIP):
             self.depose_scalar(parts, "w", "rho", charge=parts.Args["charge"])
         self.postproc_depose_scalar("rho")
 
-    def depose_currents(self, species=[], push_half=False):
+    def depose_currents(self, species=[], push_half=False, half_in=False):
         """grid.py:33-54: J of every mobile species.  With push_half, each
         mobile species is also pushed by dt/2 and keyed on this grid in the
-        same pass, as push_coords('half', grid=self) would do next."""
+        same pass, as push_coords('half', grid=self) would do next.  With
+        half_in, the deposit reads the positions from DataDev 'xh', 'yh',
+        'zh' (left there by gather_and_push(prepush=True)) and the push writes
+        x, y, z."""
         for m in range(self.Args["M"] + 1):
             for c in self.Args["vec_comps"]:
                 self.set_to(self.DataDev["J" + c + "_m" + str(m)], 0)
@@ -41,19 +44,23 @@ class Grid(GridMethodsHIP):
                 continue
             if push_half:
                 self.depose_vector_push(parts, momenta, ["g_inv", "w"], "J",
-                                        charge=parts.Args["charge"], dt=parts.Args["dt_2"])
+                                        charge=parts.Args["charge"], dt=parts.Args["dt_2"],
+                                        half_in=half_in)
             else:
                 self.depose_vector(parts, momenta, ["g_inv", "w"], "J", charge=parts.Args["charge"])
         self.postproc_depose_vector("J")
 
-    def gather_and_push(self, species=[]):
-        """grid.py:56-66."""
+    def gather_and_push(self, species=[], prepush=False):
+        """grid.py:56-66.  With prepush (consecutive steps with nothing in
+        between), each mobile species' next first half push and its sort keys
+        on this grid are computed in the same pass, into DataDev 'xh', 'yh',
+        'zh' and 'indx_in_cell'; x, y, z keep the step's positions."""
         for fld in ("E", "B"):
             self.preproc_project_vec(fld)
         for parts in species:
             if "Immobile" in parts.Args:
                 continue
-            self._gather_and_push(parts, ["E", "B"])
+            self._gather_and_push(parts, ["E", "B"], prepush=prepush)
 
     def _process_configs(self, configs_in):
         """grid.py:68-103."""

@@ -103,7 +103,7 @@ class GridMethodsHIP(GenericMethodsHIP):
                   ptr(d[factors[1]]), int(charge), self._grid(),
                   self._planes_arr([vec_fld + c for c in comps]), ptr(ws), ws.numel())
 
-    def depose_vector_push(self, parts, vec, factors, vec_fld, charge, dt):
+    def depose_vector_push(self, parts, vec, factors, vec_fld, charge, dt, half_in=False):
         """depose_vector, then the species' next coordinate push by dt and
         the cell keys of its next sort on this grid, in one pass (the
         push_coords + key pass that follow depose_currents in the step,
@@ -116,8 +116,10 @@ class GridMethodsHIP(GenericMethodsHIP):
         parts._ensure_sort_buffers(self)
         d, ws = parts.DataDev, self._dep_ws()
         comps = self.Args["vec_comps"]
-        self.call("chm_deposit_vector_push", ptr(d["sort_indx"]), ptr(d["cell_offset"]), ptr(d["x"]),
-                  ptr(d["y"]), ptr(d["z"]), *(ptr(d[v]) for v in vec), ptr(d[factors[0]]),
+        src = ("xh", "yh", "zh") if half_in else ("x", "y", "z")
+        self.call("chm_deposit_vector_push", ptr(d["sort_indx"]), ptr(d["cell_offset"]),
+                  *(ptr(d[k]) for k in src), ptr(d["x"]), ptr(d["y"]), ptr(d["z"]),
+                  *(ptr(d[v]) for v in vec), ptr(d[factors[0]]),
                   ptr(d[factors[1]]), int(charge), dt, parts.Args["Np"], ptr(d["indx_in_cell"]),
                   self._grid(), self._planes_arr([vec_fld + c for c in comps]), ptr(ws), ws.numel())
         parts._keys_for = self
@@ -143,15 +145,25 @@ class GridMethodsHIP(GenericMethodsHIP):
         names = [vec_fld + c for c in self.Args["vec_comps"]]
         self.call("chm_warp_axis", self._planes_arr(names), len(names), a["Nx"], a["Nr"], a["M"])
 
-    def _gather_and_push(self, parts, flds):
+    def _gather_and_push(self, parts, flds, prepush=False):
         """grid_methods_cl.py:168-192 (visits the sorted slots, as the
-        reference kernel does; the species must be sorted for this grid)."""
+        reference kernel does; the species must be sorted for this grid).
+        prepush: see Grid.gather_and_push."""
         if parts.Args["Np"] == 0:
             return
         d = parts.DataDev
         comps = self.Args["vec_comps"]
         E = self._planes_arr([flds[0] + c for c in comps])
         B = self._planes_arr([flds[1] + c for c in comps])
+        if prepush:
+            parts.ensure_prepush_buffers(self)
+            self.call("chm_gather_push_pre", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]),
+                      ptr(d["py"]), ptr(d["pz"]), ptr(d["g_inv"]), parts.Args["Np"],
+                      ptr(d["sort_indx"]), ptr(d["cell_offset"]), ptr(d["Np_stay"]),
+                      ptr(d.get("pid")), parts.Args["FactorPush"], parts.Args["dt_2"],
+                      ptr(d["xh"]), ptr(d["yh"]), ptr(d["zh"]), ptr(d["indx_in_cell"]),
+                      self._grid(), E, B)
+            return
         args = (ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]), ptr(d["py"]), ptr(d["pz"]),
                 ptr(d["g_inv"]), parts.Args["Np"], ptr(d["sort_indx"]), ptr(d["Np_stay"]))
         if "pid" in d:  # storage reordered: the skip uses the unreordered index

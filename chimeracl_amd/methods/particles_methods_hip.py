@@ -222,6 +222,26 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         self.flag_sorted = False
         self.version += 1
 
+    def ensure_prepush_buffers(self, grid):
+        """DataDev 'xh', 'yh', 'zh' (positions of the next first half push,
+        written by a pre-pushing gather) and the sort buffers of grid."""
+        self._ensure_sort_buffers(grid)
+        d = self.DataDev
+        for k in ("xh", "yh", "zh"):
+            if k not in d or d[k].numel() != self.Args["Np"]:
+                d[k] = torch.empty_like(d["x"])
+
+    def use_prepushed(self, grid):
+        """Takes the positions and keys a pre-pushing gather left (xh, yh,
+        zh, indx_in_cell) as this step's push_coords('half', grid): the next
+        sort_parts(grid) sorts those keys and the J deposit reads xh, yh, zh
+        (Grid.depose_currents(half_in=True))."""
+        if self.Args["Np"] == 0 or "Immobile" in self.Args:
+            return
+        self._keys_for = grid
+        self.flag_sorted = False
+        self.version += 1
+
     def ensure_pid(self):
         """DataDev['pid'] (the index of each particle without reordering),
         the identity if the storage has not been reordered yet."""
@@ -229,7 +249,7 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         if "pid" not in d and self.Args["Np"] > 0:
             d["pid"] = torch.arange(self.Args["Np"], dtype=torch.int32, device=d["x"].device)
 
-    def reorder_parts(self):
+    def reorder_parts(self, extra=()):
         """Storage order only, no counterpart in the reference: every
         component of all Np particles is permuted into the current sorted
         order with the align kernel (none dropped), and sort_indx becomes the
@@ -237,8 +257,9 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         Values, cell offsets and Np_stay are unchanged; particle indices
         change, and DataDev['pid'] keeps each particle's index in the
         numbering without reordering (the gather's skip of particles with
-        index >= Np_stay uses it, so the same particles are skipped).  Needs
-        a current sort (flag_sorted)."""
+        index >= Np_stay uses it, so the same particles are skipped).  extra:
+        further per-particle DataDev arrays to permute along (the pre-pushed
+        positions and their keys).  Needs a current sort (flag_sorted)."""
         Np = self.Args["Np"]
         if Np == 0 or not self.flag_sorted:
             return
@@ -250,13 +271,14 @@ class ParticleMethodsHIP(GenericMethodsHIP):
                                torch.arange(Np, dtype=torch.int32, device=dev))
         scr, iota = self._align_buf
         srt = d["sort_indx"]
-        for comp in self._arg_names() + ["pid"]:
+        for comp in self._arg_names() + ["pid"] + list(extra):
             a = d[comp]
             tmp = scr.view(torch.uint8)[: a.numel() * a.element_size()].view(a.dtype)
             self.call_raw("chm_align_elems", ptr(srt), Np, ptr(a), ptr(tmp), a.element_size())
             a.copy_(tmp)
         srt.copy_(iota)
-        self._keys_for = None
+        if not extra:
+            self._keys_for = None
         self.version += 1
 
     def free_mp(self):

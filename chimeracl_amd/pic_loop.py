@@ -42,7 +42,8 @@ class PIC_loop:
         self.timit = timit
         self.it = 0
         self.graph = graph
-        self._graph = None
+        self._graphs = {}    # (consume, produce) -> captured step
+        self._pre = False    # the last step left the next first half push
         self.Timer = {s: 0.0 for s in loop_steps}
         self._ev = []
 
@@ -66,7 +67,19 @@ class PIC_loop:
             self._ev = []
 
     # ---------------------------------------------------------------- step
-    def _body(self):
+    def _fused(self):
+        """One solver: its J deposit also applies the second half push and
+        the sort keys (the same positions and keys as push_coords).  Its
+        writes go through sort_indx, so it pays while the particles' memory
+        order follows their cells: always in FP64, in FP32 with align_every
+        (measured on MI355X, 16.6M electrons)."""
+        return len(self.solvers) == 1 and (self.align_every > 0 or
+                                           not self.mainsolver.comm.mixed)
+
+    def _body(self, consume=False, produce=False):
+        """One step.  consume: the previous step's gather left this step's
+        first half push (positions xh, yh, zh and their keys); produce: this
+        step's gather leaves the next one's (Grid.gather_and_push(prepush))."""
         self.timer_start()
         for frame in self.frames:
             if np.mod(self.it, frame.Args["Steps"]) == 0:
@@ -75,21 +88,18 @@ class PIC_loop:
         self.timer_record("frame")
         for parts in self.species:
             self.timer_start()
-            parts.push_coords(mode="half", grid=self.mainsolver)
+            if consume:
+                parts.use_prepushed(self.mainsolver)
+            else:
+                parts.push_coords(mode="half", grid=self.mainsolver)
             self.timer_record("push-x")
             self.timer_start()
             parts.sort_parts(grid=self.mainsolver)
             self.timer_record("sort")
-        # one solver: its J deposit also applies the second half push and
-        # the sort keys (the same positions and keys as push_coords).  Its
-        # writes go through sort_indx, so it pays while the particles' memory
-        # order follows their cells: always in FP64, in FP32 with align_every
-        # (measured on MI355X, 16.6M electrons)
-        fused = len(self.solvers) == 1 and (self.align_every > 0 or
-                                            not self.mainsolver.comm.mixed)
+        fused = self._fused()
         self.timer_start()
         for solver in self.solvers:
-            solver.depose_currents(species=self.species, push_half=fused)
+            solver.depose_currents(species=self.species, push_half=fused, half_in=consume)
         self.timer_record("depose")
         for parts in self.species:
             if not fused:
@@ -119,27 +129,58 @@ class PIC_loop:
             solver.fb_transform(vects=["E", "B"], dir=1)
             self.timer_record("transform")
             self.timer_start()
-            solver.gather_and_push(species=self.species)
+            solver.gather_and_push(species=self.species, prepush=produce)
             self.timer_record("gather + push-p")
 
     def step(self):
-        self.pre_step()
-        if self.graph and self._graph is None and self.it >= 1 and not self.frames and not self.timit:
-            self._capture()
-        if self._graph is not None:
-            self._graph.replay()
+        """One PIC step (pic_loop.py:57-142)."""
+        return self._step(self._pre, False)
+
+    def run(self, nsteps):
+        """nsteps consecutive steps.  Inside the call nothing else touches the
+        particles between two steps, so (one solver, the fused deposit) each
+        step's gather also applies the next step's first half push and its
+        sort keys (chm_gather_push_pre_*), except before a step with a frame
+        event or a diagnostics record, and before the call returns: the state
+        after run(n) is the state after n calls of step(), bit for bit."""
+        for i in range(nsteps):
+            produce = (i < nsteps - 1 and self._fused() and self._plain_step(self.it + 1))
+            self._step(self._pre, produce)
+        return self.it
+
+    def _plain_step(self, it):
+        """No frame event and no diagnostics record at step it."""
+        for frame in self.frames:
+            if np.mod(it, frame.Args["Steps"]) == 0:
+                return False
+        for diag in self.diags:
+            if np.mod(it, diag.Args.get("Interval", 1)) == 0:
+                return False
+        return True
+
+    def _step(self, consume, produce):
+        self.pre_step(consume)
+        key = (consume, produce)
+        if (self.graph and key not in self._graphs and self.it >= 1 and not self.frames
+                and not self.timit):
+            self._capture(key)
+        g = self._graphs.get(key) if self.graph else None
+        if g is not None:
+            g.replay()
             for parts in self.species:
                 if parts.Args["Np"] > 0 and "Immobile" not in parts.Args:
                     parts.Args.set_lazy("Np_stay", (lambda t: lambda: int(t.item()))(parts.DataDev["Np_stay"]))
         else:
-            self._body()
+            self._body(consume, produce)
             self._flush_timers()
+        self._pre = produce
         self.it += 1
         return self.it
 
-    def pre_step(self):
+    def pre_step(self, consume=False):
         """What a step does before its kernels: diagnostics records, then
-        (align_every) the reorder of the mobile species."""
+        (align_every) the reorder of the mobile species (with the pre-pushed
+        positions and keys when the step consumes them)."""
         for diag in self.diags:
             diag.make_record(self.it)
         if self.align_every:
@@ -150,17 +191,21 @@ class PIC_loop:
                     # creates it
                     parts.ensure_pid()
                     if self.it > 0 and self.it % self.align_every == 0:
-                        parts.reorder_parts()
+                        parts.reorder_parts(
+                            ("xh", "yh", "zh", "indx_in_cell") if consume else ())
 
-    def _capture(self):
+    def _capture(self, key):
         # one eager step has run, so every buffer exists; ions stay sorted
         # (they are never pushed), electrons are sorted twice per step
+        for parts in self.species:
+            if key[1] and "Immobile" not in parts.Args and parts.Args["Np"] > 0:
+                parts.ensure_prepush_buffers(self.mainsolver)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
             with torch.cuda.graph(g, stream=s):
-                self._body()
+                self._body(*key)
         torch.cuda.current_stream().wait_stream(s)
-        self._graph = g
+        self._graphs[key] = g

@@ -244,28 +244,32 @@ int chm_deposit_vector_f32(const uint32_t *sort_indx,
 /* chm_deposit_vector_* followed by the step's next coordinate push and the
  * cell keys of its sort (pic_loop.py:82-91: depose_currents, then
  * push_coords('half') and the key pass of sort_parts), in one pass over the
- * particles: J from the positions before the push, then every one of the Np
- * particles is pushed by dt (x += u*dt*g_inv, chm_push_x_*'s operations)
- * and indx_in_cell receives the keys chm_push_x_keys_* would write, ready
- * for chm_sort_from_keys.  sort_indx / cell_offset must be the current
- * sort of these particles on grid g. */
+ * particles: J from the positions xin, yin, zin, then every one of the Np
+ * particles is pushed by dt (x = xin + u*dt*g_inv, chm_push_x_*'s
+ * operations) into x, y, z (which may be xin, yin, zin), and indx_in_cell
+ * receives the keys chm_push_x_keys_* would write, ready for
+ * chm_sort_from_keys.  sort_indx / cell_offset must be the current sort of
+ * these particles on grid g. */
 int chm_deposit_vector_push_f64(const uint32_t *sort_indx,
-                                const uint32_t *cell_offset, double *x,
-                                double *y, double *z, const double *ux,
-                                const double *uy, const double *uz,
-                                const double *g_inv, const double *w,
-                                int32_t charge, double dt, int64_t Np,
-                                uint32_t *indx_in_cell, const chm_grid_t *g,
-                                const chm_planes_t *J, void *ws,
-                                size_t ws_bytes, void *stream);
-int chm_deposit_vector_push_f32(const uint32_t *sort_indx,
-                                const uint32_t *cell_offset, float *x, float *y,
-                                float *z, const float *ux, const float *uy,
-                                const float *uz, const float *g_inv,
-                                const float *w, int32_t charge, float dt,
+                                const uint32_t *cell_offset, const double *xin,
+                                const double *yin, const double *zin,
+                                double *x, double *y, double *z,
+                                const double *ux, const double *uy,
+                                const double *uz, const double *g_inv,
+                                const double *w, int32_t charge, double dt,
                                 int64_t Np, uint32_t *indx_in_cell,
                                 const chm_grid_t *g, const chm_planes_t *J,
                                 void *ws, size_t ws_bytes, void *stream);
+int chm_deposit_vector_push_f32(const uint32_t *sort_indx,
+                                const uint32_t *cell_offset, const float *xin,
+                                const float *yin, const float *zin, float *x,
+                                float *y, float *z, const float *ux,
+                                const float *uy, const float *uz,
+                                const float *g_inv, const float *w,
+                                int32_t charge, float dt, int64_t Np,
+                                uint32_t *indx_in_cell, const chm_grid_t *g,
+                                const chm_planes_t *J, void *ws,
+                                size_t ws_bytes, void *stream);
 /* The node pass of a deposit alone: adds to the planes the per-cell sums a
  * previous chm_deposit_scalar_* (vector = 0) or chm_deposit_vector_*
  * (vector = 1) call left in ws, in the same order.  For a species whose
@@ -328,6 +332,33 @@ int chm_gather_push_ids_f32(const float *x, const float *y, const float *z,
                             int64_t Np, const uint32_t *sort_indx,
                             const uint32_t *np_stay, const uint32_t *pid,
                             float factor_push, const chm_grid_t *g,
+                            const chm_planes_t *E, const chm_planes_t *B,
+                            void *stream);
+/* chm_gather_push_ids_* (pid may be NULL) followed by the next step's first
+ * coordinate push and the keys of its sort (pic_loop.py:134 gather_and_push,
+ * then pic_loop.py:70-77 push_coords('half') + the key pass of sort_parts of
+ * the next step), for consecutive steps with nothing in between: every one of
+ * the Np particles (the ones the gather skips and those outside the domain
+ * included) is pushed by dt from x, y, z with the momenta and g_inv it now
+ * holds, into xh, yh, zh (x, y, z are left as they are), and indx_in_cell
+ * receives the keys of those positions, ready for chm_sort_from_keys.
+ * cell_offset: the current sort's offsets (as sort_indx). */
+int chm_gather_push_pre_f64(const double *x, const double *y, const double *z,
+                            double *px, double *py, double *pz, double *g_inv,
+                            int64_t Np, const uint32_t *sort_indx,
+                            const uint32_t *cell_offset, const uint32_t *np_stay,
+                            const uint32_t *pid, double factor_push, double dt,
+                            double *xh, double *yh, double *zh,
+                            uint32_t *indx_in_cell, const chm_grid_t *g,
+                            const chm_planes_t *E, const chm_planes_t *B,
+                            void *stream);
+int chm_gather_push_pre_f32(const float *x, const float *y, const float *z,
+                            float *px, float *py, float *pz, float *g_inv,
+                            int64_t Np, const uint32_t *sort_indx,
+                            const uint32_t *cell_offset, const uint32_t *np_stay,
+                            const uint32_t *pid, float factor_push, float dt,
+                            float *xh, float *yh, float *zh,
+                            uint32_t *indx_in_cell, const chm_grid_t *g,
                             const chm_planes_t *E, const chm_planes_t *B,
                             void *stream);
 
