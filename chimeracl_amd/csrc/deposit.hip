@@ -58,12 +58,14 @@ struct PlanePtrs {
 // coordinate push (x += u dt g_inv, push_x's operations) and writes the
 // pushed position (to xo, which may be x) and its cell key (the keys of the
 // following sort); the sums use the positions before the push.
-template <int M, int KIND> struct DepCfg {
+template <typename T, int M, int KIND> struct DepCfg {
   static constexpr bool VEC = KIND != 0;
   static constexpr int NK = VEC ? 3 : 1;              // components
   static constexpr int TPC = 4 * NK;                   // lanes per cell
   static constexpr int CELLS = 64 / TPC;               // cells per wave
-  static constexpr int CAP = VEC ? 192 : 512;          // staged slots per chunk
+  // staged slots per chunk: smaller chunks (more chunks per wave, fewer
+  // registers and LDS, more waves) measured faster in FP64 at C2
+  static constexpr int CAP = sizeof(T) == 8 ? (VEC ? 96 : 128) : (VEC ? 192 : 256);
   static constexpr int NV = 4 + 2 * M + (VEC ? 3 : 0); // staged values per particle
 };
 // Lanes walk their own cell's slots, ~ppc apart: one pad element per 32
@@ -78,7 +80,7 @@ __global__ void __launch_bounds__(64)
                 const T *__restrict__ g_inv, const T *__restrict__ w, T charge, int Nx, int Nr,
                 T xmin, T dx_inv, T rmin, T dr_inv, T *__restrict__ part, T dt,
                 uint32_t *__restrict__ key_out, T *xo, T *yo, T *zo) {
-  typedef DepCfg<M, KIND> Cfg;
+  typedef DepCfg<T, M, KIND> Cfg;
   constexpr bool VEC = Cfg::VEC;
   constexpr int CAP = Cfg::CAP, NV = Cfg::NV, TPC = Cfg::TPC;
   constexpr int MM = M > 0 ? M : 1;
@@ -343,7 +345,7 @@ static void launch_deposit(const uint32_t *sort_indx, const uint32_t *cell_offse
                            const PlanePtrs &pp, T *part, bool pass1, hipStream_t st,
                            const DepPush<T> &push) {
   int64_t ncell = (int64_t)(g->Nx - 1) * (g->Nr - 1);
-  const unsigned nb = chm_blocks(ncell, DepCfg<M, KIND>::CELLS);
+  const unsigned nb = chm_blocks(ncell, DepCfg<T, M, KIND>::CELLS);
   if (pass1 && KIND == 1 && push.keys) {
     k_cell_sums<T, M, KIND, true><<<nb, 64, 0, st>>>(
         sort_indx, cell_offset, x, y, z, ux, uy, uz, g_inv, w, charge, g->Nx, g->Nr, (T)g->xmin,
