@@ -81,6 +81,8 @@ _SIGS = {
     "chm_deposit_add_staged_f32": [P, R(Grid), R(Planes), i32, P, sz, P],
     "chm_postproc_deposit_f64": [R(Planes), i32, P, i32, i32, i32, P],
     "chm_postproc_deposit_f32": [R(Planes), i32, P, i32, i32, i32, P],
+    "chm_zero_planes_f64": [R(Planes), i32, i32, i32, i32, P],
+    "chm_zero_planes_f32": [R(Planes), i32, i32, i32, i32, P],
     "chm_warp_axis_f64": [R(Planes), i32, i32, i32, i32, P],
     "chm_warp_axis_f32": [R(Planes), i32, i32, i32, i32, P],
     "chm_gather_push_f64": [P, P, P, P, P, P, P, i64, P, P, f64, R(Grid), R(Planes), R(Planes), P],

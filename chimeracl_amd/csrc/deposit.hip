@@ -597,6 +597,38 @@ static int warp(const chm_planes_t *flds, int32_t nflds, int32_t Nx, int32_t Nr,
   return CHM_OK;
 }
 
+// All planes of nflds fields set to zero in one launch (the set_to(.., 0)
+// calls before a deposit, grid.py:23-54).
+template <typename T> __global__ void k_zero_planes(PlaneList pl, int64_t n_real) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int pi = blockIdx.y;
+  const int64_t n = pl.cplx[pi] ? 2 * n_real : n_real;
+  if (e < n) ((T *)pl.p[pi])[e] = (T)0;
+}
+
+template <typename T>
+static int zero_planes(const chm_planes_t *flds, int32_t nflds, int32_t Nx, int32_t Nr, int32_t M,
+                       void *stream) {
+  if (!flds || nflds <= 0 || M < 0 || M > CHM_MAXM || Nr < 2 || Nx < 1) return CHM_EINVAL;
+  PlaneList pl;
+  const int np = fill_plane_list(&pl, flds, nflds, M);
+  if (np < 0) return CHM_EINVAL;
+  const int64_t n = (int64_t)Nr * Nx;
+  dim3 grid(chm_blocks(2 * n, 256), np);
+  k_zero_planes<T><<<grid, 256, 0, (hipStream_t)stream>>>(pl, n);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_zero_planes_f64(const chm_planes_t *flds, int32_t nflds, int32_t Nx, int32_t Nr,
+                                   int32_t M, void *stream) {
+  return zero_planes<double>(flds, nflds, Nx, Nr, M, stream);
+}
+extern "C" int chm_zero_planes_f32(const chm_planes_t *flds, int32_t nflds, int32_t Nx, int32_t Nr,
+                                   int32_t M, void *stream) {
+  return zero_planes<float>(flds, nflds, Nx, Nr, M, stream);
+}
+
 extern "C" int chm_postproc_deposit_f64(const chm_planes_t *flds, int32_t nflds,
                                         const double *dV_inv, int32_t Nx, int32_t Nr, int32_t M,
                                         void *stream) {

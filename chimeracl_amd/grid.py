@@ -22,8 +22,7 @@ class Grid(GridMethodsHIP):
 
     def depose_charge(self, species=[]):
         """grid.py:23-31: rho of every species (immobile ones included)."""
-        for m in range(self.Args["M"] + 1):
-            self.set_to(self.DataDev["rho_m" + str(m)], 0)
+        self.zero_planes(["rho"])
         for parts in species:
             self.depose_scalar(parts, "w", "rho", charge=parts.Args["charge"])
         self.postproc_depose_scalar("rho")
@@ -35,9 +34,7 @@ class Grid(GridMethodsHIP):
         half_in, the deposit reads the positions from DataDev 'xh', 'yh',
         'zh' (left there by gather_and_push(prepush=True)) and the push writes
         x, y, z."""
-        for m in range(self.Args["M"] + 1):
-            for c in self.Args["vec_comps"]:
-                self.set_to(self.DataDev["J" + c + "_m" + str(m)], 0)
+        self.zero_planes(["J" + c for c in self.Args["vec_comps"]])
         momenta = ["p" + c for c in self.Args["vec_comps"]]
         for parts in species:
             if "Immobile" in parts.Args:

@@ -126,6 +126,11 @@ class GridMethodsHIP(GenericMethodsHIP):
         parts.flag_sorted = False
         parts.version += 1
 
+    def zero_planes(self, names):
+        """set_to(.., 0) of every mode plane of the named fields, one launch."""
+        a = self.Args
+        self.call("chm_zero_planes", self._planes_arr(names), len(names), a["Nx"], a["Nr"], a["M"])
+
     def _postproc(self, names):
         a = self.Args
         self.call("chm_postproc_deposit", self._planes_arr(names), len(names),

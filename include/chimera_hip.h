@@ -291,6 +291,13 @@ int chm_postproc_deposit_f32(const chm_planes_t *flds, int32_t nflds,
                              const double *dV_inv, int32_t Nx, int32_t Nr,
                              int32_t M, void *stream);
 /* Ghost row: row0 = row1 (m=0), row0 = -row1 (m>=1), for nflds fields. */
+/* Grid.set_to(plane, 0) of every plane (modes 0..M) of nflds fields in one
+ * launch, as Grid.depose_charge / depose_currents do before depositing
+ * (grid.py:23-54). */
+int chm_zero_planes_f64(const chm_planes_t *flds, int32_t nflds, int32_t Nx,
+                        int32_t Nr, int32_t M, void *stream);
+int chm_zero_planes_f32(const chm_planes_t *flds, int32_t nflds, int32_t Nx,
+                        int32_t Nr, int32_t M, void *stream);
 int chm_warp_axis_f64(const chm_planes_t *flds, int32_t nflds, int32_t Nx,
                       int32_t Nr, int32_t M, void *stream);
 int chm_warp_axis_f32(const chm_planes_t *flds, int32_t nflds, int32_t Nx,
