@@ -220,20 +220,19 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_final(uint32_t *__restrict__ i
 // Np_stay = cell_offset[nbins - 1], the start of the out-of-domain bin.
 // Slot s starts the cells key[s-1] < c <= key[s] (key[-1] = -1,
 // key[Np] = nbins); these intervals partition 0..nbins.  A slot writes the
-// first OFF_RUN cells of its interval; k_offsets_gaps then finds the rest
-// (cells of long empty stretches, still holding the fill value) by bisection.
+// cells of its interval, at most OFF_RUN of them, and raises *trunc when its
+// interval is longer (a long stretch of empty cells); then k_offsets_gaps
+// recomputes every cell by bisection.  *trunc is cleared earlier in the same
+// stream (k_rs_colsum of the first pass).
 #define OFF_RUN 32
-__global__ void k_offsets_fill(uint32_t *__restrict__ cell_offset, int64_t n) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < n) cell_offset[c] = 0xFFFFFFFFu;
-}
 
 // OFF_PER consecutive slots per lane (two 16-byte key loads).
 #define OFF_PER 8
 __global__ void __launch_bounds__(256) k_offsets_bounds(const uint32_t *__restrict__ skey, int64_t Np,
                                                         int64_t nbins,
                                                         uint32_t *__restrict__ cell_offset,
-                                                        uint32_t *__restrict__ np_stay) {
+                                                        uint32_t *__restrict__ np_stay,
+                                                        uint32_t *__restrict__ trunc) {
   const int64_t s0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * OFF_PER;
   if (s0 > Np) return;
   uint32_t kv[OFF_PER];
@@ -253,7 +252,10 @@ __global__ void __launch_bounds__(256) k_offsets_bounds(const uint32_t *__restri
     const int64_t cur = s < Np ? (int64_t)kv[i] : nbins;
     if (cur > prev) {
       int64_t hi = cur;
-      if (hi > prev + OFF_RUN) hi = prev + OFF_RUN;
+      if (hi > prev + OFF_RUN) {
+        hi = prev + OFF_RUN;
+        *trunc = 1u;
+      }
       for (int64_t c = prev + 1; c <= hi; c++) {
         cell_offset[c] = (uint32_t)s;
         if (c == nbins - 1) *np_stay = (uint32_t)s;
@@ -264,9 +266,10 @@ __global__ void __launch_bounds__(256) k_offsets_bounds(const uint32_t *__restri
 }
 
 __global__ void k_offsets_gaps(const uint32_t *__restrict__ skey, int64_t Np, int64_t nbins,
-                               uint32_t *__restrict__ cell_offset, uint32_t *__restrict__ np_stay) {
+                               uint32_t *__restrict__ cell_offset, uint32_t *__restrict__ np_stay,
+                               const uint32_t *__restrict__ trunc) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c > nbins || cell_offset[c] != 0xFFFFFFFFu) return;
+  if (*trunc == 0u || c > nbins) return;
   int64_t lo = 0, hi = Np;
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
@@ -276,6 +279,8 @@ __global__ void k_offsets_gaps(const uint32_t *__restrict__ skey, int64_t Np, in
   cell_offset[c] = (uint32_t)lo;
   if (c == nbins - 1) *np_stay = (uint32_t)lo;
 }
+
+__global__ void k_flag_set(uint32_t *__restrict__ flag, uint32_t v) { *flag = v; }
 
 // ------------------------------------------------------------------ radix sort
 // Digits of nbits <= 2^RB buckets; counts[tile * nd + d] (one contiguous
@@ -417,8 +422,10 @@ __global__ void __launch_bounds__(RS_BS) k_rs_downsweep(const uint32_t *__restri
 #define RS_TC 16
 __global__ void __launch_bounds__(256) k_rs_colsum(const uint32_t *__restrict__ counts, int64_t nblk,
                                                    int nbits, int64_t nch,
-                                                   uint32_t *__restrict__ csumT) {
+                                                   uint32_t *__restrict__ csumT,
+                                                   uint32_t *__restrict__ clear) {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;  // ch * nd + d
+  if (clear && e == 0) *clear = 0u;
   const int nd = 1 << nbits;
   if (e >= nch * nd) return;
   const int64_t ch = e >> nbits;
@@ -485,7 +492,7 @@ static int key_bits(int64_t nbins) {
 
 // ------------------------------------------------------------------ driver
 struct SortWs {
-  uint32_t *counts, *offs, *csum, *cpre, *bsum2, *kA, *kB, *vA, *vB;
+  uint32_t *counts, *offs, *csum, *cpre, *bsum2, *kA, *kB, *vA, *vB, *trunc;
   int64_t nblk, nch;
 };
 
@@ -504,6 +511,7 @@ static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
   size_t o_bsum2 = off; off += chm_align(nscan2 * 4);
   size_t o_k = off; off += 2 * chm_align(Np * 4);
   size_t o_v = off; off += 2 * chm_align(Np * 4);
+  size_t o_t = off; off += chm_align(4);
   if (w) {
     char *b = (char *)base;
     w->counts = (uint32_t *)(b + o_counts);
@@ -515,6 +523,7 @@ static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
     w->kB = (uint32_t *)(b + o_k + chm_align(Np * 4));
     w->vA = (uint32_t *)(b + o_v);
     w->vB = (uint32_t *)(b + o_v + chm_align(Np * 4));
+    w->trunc = (uint32_t *)(b + o_t);
     w->nblk = nblk;
     w->nch = nch;
   }
@@ -575,7 +584,8 @@ static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_
         k_rs_upsweep<8><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
       else
         k_rs_upsweep<11><<<(unsigned)w.nblk, RS_BS, 0, st>>>(kin, Np, shift, nb, w.counts, w.nblk);
-      k_rs_colsum<<<chm_blocks(ncs, 256), 256, 0, st>>>(w.counts, w.nblk, nb, w.nch, w.csum);
+      k_rs_colsum<<<chm_blocks(ncs, 256), 256, 0, st>>>(w.counts, w.nblk, nb, w.nch, w.csum,
+                                                         p == 0 ? w.trunc : nullptr);
       scan_u32(w.csum, ncs, w.cpre, w.bsum2, nsc, st);
       k_rs_coloffs<<<chm_blocks(ncs, 256), 256, 0, st>>>(w.counts, w.nblk, nb, w.nch, w.cpre,
                                                           w.offs);
@@ -591,10 +601,11 @@ static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_
     }
     skey = kin;
   }
-  k_offsets_fill<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(cell_offset, nbins + 1);
+  if (Np == 0) k_flag_set<<<1, 1, 0, st>>>(w.trunc, 1u);  // no pass cleared it: bisect
   k_offsets_bounds<<<chm_blocks(Np / OFF_PER + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset,
-                                                                       np_stay);
-  k_offsets_gaps<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset, np_stay);
+                                                                       np_stay, w.trunc);
+  k_offsets_gaps<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset, np_stay,
+                                                             w.trunc);
   CHM_CHECK_LAUNCH();
   return CHM_OK;
 }
