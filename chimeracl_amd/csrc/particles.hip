@@ -220,11 +220,15 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_final(uint32_t *__restrict__ i
 // Np_stay = cell_offset[nbins - 1], the start of the out-of-domain bin.
 // Slot s starts the cells key[s-1] < c <= key[s] (key[-1] = -1,
 // key[Np] = nbins); these intervals partition 0..nbins.  A slot writes the
-// cells of its interval, at most OFF_RUN of them, and raises *trunc when its
-// interval is longer (a long stretch of empty cells); then k_offsets_gaps
-// recomputes every cell by bisection.  *trunc is cleared earlier in the same
-// stream (k_rs_colsum of the first pass).
+// cells of its interval, at most OFF_RUN of them; the rest of a longer
+// interval (a long stretch of empty cells, e.g. the invalid last row before
+// the out-of-domain bin) goes to a short list that k_offsets_gaps fills.  If
+// the list overflows, k_offsets_gaps recomputes every cell by bisection.
+// gl[0] = entries, gl[1] = overflow; both cleared earlier in the same stream
+// (k_rs_colsum of the first pass); entries (s, first cell, last cell) from
+// gl[2].
 #define OFF_RUN 32
+#define OFF_LIST 1024
 
 // OFF_PER consecutive slots per lane (two 16-byte key loads).
 #define OFF_PER 8
@@ -232,7 +236,7 @@ __global__ void __launch_bounds__(256) k_offsets_bounds(const uint32_t *__restri
                                                         int64_t nbins,
                                                         uint32_t *__restrict__ cell_offset,
                                                         uint32_t *__restrict__ np_stay,
-                                                        uint32_t *__restrict__ trunc) {
+                                                        uint32_t *__restrict__ gl) {
   const int64_t s0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * OFF_PER;
   if (s0 > Np) return;
   uint32_t kv[OFF_PER];
@@ -254,7 +258,14 @@ __global__ void __launch_bounds__(256) k_offsets_bounds(const uint32_t *__restri
       int64_t hi = cur;
       if (hi > prev + OFF_RUN) {
         hi = prev + OFF_RUN;
-        *trunc = 1u;
+        const uint32_t e = atomicAdd(&gl[0], 1u);
+        if (e < OFF_LIST) {
+          gl[2 + 3 * e] = (uint32_t)s;
+          gl[3 + 3 * e] = (uint32_t)(hi + 1);
+          gl[4 + 3 * e] = (uint32_t)cur;
+        } else {
+          gl[1] = 1u;
+        }
       }
       for (int64_t c = prev + 1; c <= hi; c++) {
         cell_offset[c] = (uint32_t)s;
@@ -265,22 +276,38 @@ __global__ void __launch_bounds__(256) k_offsets_bounds(const uint32_t *__restri
   }
 }
 
+// Fixed grid; every lane strides over the listed cells (or, after an
+// overflow of the list, over all cells with a bisection each).
 __global__ void k_offsets_gaps(const uint32_t *__restrict__ skey, int64_t Np, int64_t nbins,
                                uint32_t *__restrict__ cell_offset, uint32_t *__restrict__ np_stay,
-                               const uint32_t *__restrict__ trunc) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (*trunc == 0u || c > nbins) return;
-  int64_t lo = 0, hi = Np;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if ((int64_t)skey[mid] < c) lo = mid + 1;
-    else hi = mid;
+                               const uint32_t *__restrict__ gl) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nt = (int64_t)gridDim.x * blockDim.x;
+  if (gl[1] != 0u) {
+    for (int64_t c = t0; c <= nbins; c += nt) {
+      int64_t lo = 0, hi = Np;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)skey[mid] < c) lo = mid + 1;
+        else hi = mid;
+      }
+      cell_offset[c] = (uint32_t)lo;
+      if (c == nbins - 1) *np_stay = (uint32_t)lo;
+    }
+    return;
   }
-  cell_offset[c] = (uint32_t)lo;
-  if (c == nbins - 1) *np_stay = (uint32_t)lo;
+  const uint32_t ne = gl[0] < OFF_LIST ? gl[0] : OFF_LIST;
+  for (uint32_t e = 0; e < ne; e++) {
+    const uint32_t sv = gl[2 + 3 * e];
+    const int64_t c0 = gl[3 + 3 * e], c1 = gl[4 + 3 * e];
+    for (int64_t c = c0 + t0; c <= c1; c += nt) {
+      cell_offset[c] = sv;
+      if (c == nbins - 1) *np_stay = sv;
+    }
+  }
 }
 
-__global__ void k_flag_set(uint32_t *__restrict__ flag, uint32_t v) { *flag = v; }
+__global__ void k_list_clear(uint32_t *__restrict__ gl) { gl[threadIdx.x] = 0u; }
 
 // ------------------------------------------------------------------ radix sort
 // Digits of nbits <= 2^RB buckets; counts[tile * nd + d] (one contiguous
@@ -425,7 +452,7 @@ __global__ void __launch_bounds__(256) k_rs_colsum(const uint32_t *__restrict__ 
                                                    uint32_t *__restrict__ csumT,
                                                    uint32_t *__restrict__ clear) {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;  // ch * nd + d
-  if (clear && e == 0) *clear = 0u;
+  if (clear && e < 2) clear[e] = 0u;
   const int nd = 1 << nbits;
   if (e >= nch * nd) return;
   const int64_t ch = e >> nbits;
@@ -511,7 +538,7 @@ static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
   size_t o_bsum2 = off; off += chm_align(nscan2 * 4);
   size_t o_k = off; off += 2 * chm_align(Np * 4);
   size_t o_v = off; off += 2 * chm_align(Np * 4);
-  size_t o_t = off; off += chm_align(4);
+  size_t o_t = off; off += chm_align((2 + 3 * OFF_LIST) * 4);
   if (w) {
     char *b = (char *)base;
     w->counts = (uint32_t *)(b + o_counts);
@@ -601,11 +628,10 @@ static int sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_
     }
     skey = kin;
   }
-  if (Np == 0) k_flag_set<<<1, 1, 0, st>>>(w.trunc, 1u);  // no pass cleared it: bisect
+  if (Np == 0) k_list_clear<<<1, 2, 0, st>>>(w.trunc);  // no radix pass cleared it
   k_offsets_bounds<<<chm_blocks(Np / OFF_PER + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset,
                                                                        np_stay, w.trunc);
-  k_offsets_gaps<<<chm_blocks(nbins + 1, 256), 256, 0, st>>>(skey, Np, nbins, cell_offset, np_stay,
-                                                             w.trunc);
+  k_offsets_gaps<<<256, 256, 0, st>>>(skey, Np, nbins, cell_offset, np_stay, w.trunc);
   CHM_CHECK_LAUNCH();
   return CHM_OK;
 }
