@@ -762,6 +762,9 @@ __global__ void __launch_bounds__(256)
 // bit for bit.
 #define GS_NT 256
 template <typename T, int M> struct GatherCfg {
+  // waves per SIMD the register allocation is held to at M <= 1 (FP64 6,
+  // FP32 8: the fastest C2 steps, no spills); no bound above M = 1
+  static constexpr int WPE = M <= 1 ? (sizeof(T) == 8 ? 6 : 8) : 1;
   static constexpr int NQ = 1 + 2 * M;
   static constexpr int NV = 6 * NQ;  // E and B, 3 components, all modes
   static constexpr int MN = 20480 / (2 * NV * (int)sizeof(T));
@@ -858,7 +861,7 @@ __device__ __forceinline__ void pre_push(const PrePush<T> &pp, uint32_t p, T xp,
 }
 
 template <typename T, int M, bool PRE>
-__global__ void __launch_bounds__(GS_NT)
+__global__ void __launch_bounds__(GS_NT) __attribute__((amdgpu_waves_per_eu(GatherCfg<T, M>::WPE)))
     k_gather_sorted(const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
                     T *__restrict__ px, T *__restrict__ py, T *__restrict__ pz,
                     T *__restrict__ g_inv, const uint32_t *__restrict__ sort_indx,

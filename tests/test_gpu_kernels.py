@@ -199,6 +199,68 @@ def test_sort_large_grid(env, case):
     assert p.Args["Np_stay"] == q.Args["Np_stay"]
 
 
+# (Nx, Nr) -> key bits of nbins = (Nx-1)(Nr-1)+1: 11 (two 8-bit radix
+# passes), 17, 19 and 21 (two passes of up to 11 bits) and 23 (three passes;
+# 3000 clusters overflow the list of long empty stretches of the offsets)
+SORT_GRIDS = [(64, 30), (1024, 66), (2048, 256), (4096, 500), (4096, 1100)]
+SORT_CASES = ["random", "near_sorted", "few_cells", "clusters", "outside", "tiny", "empty"]
+
+
+def _sort_keys_case(case, nbins, rng):
+    if case == "random":
+        return rng.integers(0, nbins, 400000)
+    if case == "near_sorted":  # cell order with local disorder, as between realignments
+        k = np.sort(rng.integers(0, nbins - 1, 600000))
+        return np.clip(k + rng.integers(-3, 4, k.size), 0, nbins - 1)
+    if case == "few_cells":  # buckets far larger than one chunk of the bucket pass
+        h = nbins // 2
+        return rng.choice(np.array([0, 7, h, h + 1, nbins - 2]), 300000)
+    if case == "clusters":  # 3000 clusters: long stretches of empty cells
+        c = rng.integers(0, nbins - 1, 3000)
+        return c[rng.integers(0, 3000, 50000)]
+    if case == "outside":  # most particles in the out-of-domain bin
+        k = rng.integers(0, nbins, 100000)
+        k[rng.random(k.size) < 0.9] = nbins - 1
+        return k
+    if case == "tiny":
+        return rng.integers(0, nbins, 37)
+    return np.zeros(0, np.int64)
+
+
+@pytest.mark.parametrize("gi", range(len(SORT_GRIDS)))
+def test_sort_from_keys_widths(env, gi):
+    """chm_sort_from_keys on synthetic keys for every key width the sort
+    distinguishes and every shape of key distribution: sort_indx is the
+    stable argsort, cell_offset[c] the first sorted slot with key >= c,
+    Np_stay = cell_offset[nbins - 1] (numpy as the checker)."""
+    import ctypes
+    from chimeracl_amd import _lib
+    Nx, Nr = SORT_GRIDS[gi]
+    nbins = (Nx - 1) * (Nr - 1) + 1
+    g = _lib.Grid(Nx, Nr, 1, 0, 0.0, 1.0, 0.0, 1.0)
+    rng = np.random.default_rng(11 + gi)
+    dev = torch.device("cuda")
+    for case in SORT_CASES:
+        key = _sort_keys_case(case, nbins, rng).astype(np.uint32)
+        n = key.size
+        kd = torch.from_numpy(key.view(np.int32).copy()).to(dev)
+        off = torch.full((nbins + 1,), -1, dtype=torch.int32, device=dev)
+        srt = torch.full((max(n, 1),), -1, dtype=torch.int32, device=dev)
+        nps = torch.full((1,), -1, dtype=torch.int32, device=dev)
+        wsb = _lib.lib.chm_sort_workspace_bytes(n, nbins)
+        ws = torch.empty(int(wsb), dtype=torch.uint8, device=dev)
+        _lib.call("chm_sort_from_keys", n, ctypes.byref(g), kd.data_ptr(), off.data_ptr(),
+                  srt.data_ptr(), nps.data_ptr(), ws.data_ptr(), ws.numel(),
+                  torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        ref = np.argsort(key, kind="stable")
+        got = srt.cpu().numpy().view(np.uint32)[:n]
+        assert np.array_equal(got, ref), (case, Nx, Nr)
+        ref_off = np.searchsorted(key[ref], np.arange(nbins + 1), side="left")
+        assert np.array_equal(off.cpu().numpy().view(np.uint32), ref_off), (case, Nx, Nr)
+        assert int(nps.item()) == ref_off[nbins - 1], (case, Nx, Nr)
+
+
 def test_dens_profile_bit_exact(env):
     g = env.tiny_grid()
     solver, p, s, q = _pair_particles(env, g, 5000, 3, 1)
