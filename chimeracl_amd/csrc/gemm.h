@@ -27,5 +27,6 @@ struct GemmBatch {
 
 int chm_gemm_launch(const GemmBatch &gb, int nitems, int kind, hipStream_t st);
 
-// Leading dimension of a padded DHT-type table (K rounded up to 32).
-static inline __host__ __device__ int gemm_lda(int K) { return (K + 31) / 32 * 32; }
+// Leading dimension of a padded DHT-type table (K rounded up to 64, a
+// whole number of K-steps of every product).
+static inline __host__ __device__ int gemm_lda(int K) { return (K + 63) / 64 * 64; }

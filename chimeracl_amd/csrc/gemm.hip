@@ -3,7 +3,7 @@
 //   C[M x N] = A[M x K] @ B[K x N]      (row-major, A = a DHT-type matrix)
 //
 // A is a per-mode real matrix (DHT, DHT_inv, dDHT+-) stored zero-padded to
-// [round_up(M, 64)][lda], lda = round_up(K, 32), so A tiles load without
+// [round_up(M, 64)][lda], lda = round_up(K, 64), so A tiles load without
 // bounds checks in 16-byte vectors.  B is either a real field [K][N] (mode 0),
 // or a complex field [K][Nx] read as N = 2*Nx real columns, optionally formed
 // on the fly as alpha*X + beta*Y with either operand read through the
@@ -26,40 +26,56 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+typedef unsigned short us4 __attribute__((ext_vector_type(4)));
 
 #define BM 64
 #define BN 64
 
 // ---------------------------------------------------------------- B loader
-// Element B[k][n] (zero outside K x N).
+// Elements B[k][n], B[k][n + 1] of an even column n (zero outside K x N):
+// one 2-element load of the real field, or one complex element (re, im) of
+// each operand.  bmode 0 takes the vector load when ldb is even (aligned).
+template <typename T> struct pair_t { T a, b; };
 template <typename T>
-__device__ __forceinline__ T load_b(const GemmBatch &gb, const GemmItem &it, int k, int n) {
-  if (k >= gb.K || n >= gb.N) return (T)0;
-  if (gb.bmode == 0) return ((const T *)it.X)[(int64_t)k * gb.ldb + n];
-  const int Nx = gb.Nx, x = n >> 1, h = n & 1;
-  T re = 0, im = 0;
+__device__ __forceinline__ pair_t<T> load_b2(const GemmBatch &gb, const GemmItem &it, int k, int n) {
+  pair_t<T> r = {(T)0, (T)0};
+  if (k >= gb.K || n >= gb.N) return r;
+  if (gb.bmode == 0) {
+    const T *p = (const T *)it.X + (int64_t)k * gb.ldb + n;
+    if (n + 1 < gb.N && !(gb.ldb & 1)) {
+      typedef T t2 __attribute__((ext_vector_type(2)));
+      const t2 v = *(const t2 *)p;
+      r.a = v.x;
+      r.b = v.y;
+    } else {
+      r.a = p[0];
+      if (n + 1 < gb.N) r.b = p[1];
+    }
+    return r;
+  }
+  typedef T t2 __attribute__((ext_vector_type(2)));
+  const int Nx = gb.Nx, x = n >> 1;
   {
-    int xs = it.remapX ? ((Nx - x) % Nx) : x;
-    const T *X = (const T *)it.X + 2 * ((int64_t)k * Nx + xs);
-    T a = X[0], b = X[1];
-    if (it.remapX) a = -a;  // -conj(F(-kx)): (-re, +im)
-    re = (T)it.alpha.x * a - (T)it.alpha.y * b;
-    im = (T)it.alpha.x * b + (T)it.alpha.y * a;
+    const int xs = it.remapX ? ((Nx - x) % Nx) : x;
+    const t2 v = ((const t2 *)it.X)[(int64_t)k * Nx + xs];
+    const T a = it.remapX ? -v.x : v.x, b = v.y;  // -conj(F(-kx)): (-re, +im)
+    r.a = (T)it.alpha.x * a - (T)it.alpha.y * b;
+    r.b = (T)it.alpha.x * b + (T)it.alpha.y * a;
   }
   if (it.Y) {
-    int xs = it.remapY ? ((Nx - x) % Nx) : x;
-    const T *Y = (const T *)it.Y + 2 * ((int64_t)k * Nx + xs);
-    T a = Y[0], b = Y[1];
-    if (it.remapY) a = -a;
-    re += (T)it.beta.x * a - (T)it.beta.y * b;
-    im += (T)it.beta.x * b + (T)it.beta.y * a;
+    const int xs = it.remapY ? ((Nx - x) % Nx) : x;
+    const t2 v = ((const t2 *)it.Y)[(int64_t)k * Nx + xs];
+    const T a = it.remapY ? -v.x : v.x, b = v.y;
+    r.a += (T)it.beta.x * a - (T)it.beta.y * b;
+    r.b += (T)it.beta.x * b + (T)it.beta.y * a;
   }
-  return h ? im : re;
+  return r;
 }
 
 // ---------------------------------------------------------------- f64 / f32
+// (registers held to 5 waves per SIMD: the accumulators stay in VGPRs)
 template <typename T>
-__global__ void __launch_bounds__(256) k_gemm(GemmBatch gb) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_gemm(GemmBatch gb) {
   constexpr int BK = 16;
   constexpr int AP = (sizeof(T) == 8) ? 8 : 16;  // row pad: stride = 128 B mod 256 B
   __shared__ T As[BK][BM + AP];
@@ -71,15 +87,16 @@ __global__ void __launch_bounds__(256) k_gemm(GemmBatch gb) {
   const T *A = (const T *)it.A;
   // A: thread loads row ar, k = ak..ak+3 (padded table, no checks)
   const int ar = tid >> 2, ak = (tid & 3) * 4;
-  // B: thread owns column bn and rows bk, bk+4, bk+8, bk+12
-  const int bn = tid & 63, bk = tid >> 6;
-  T ra[4], rb[4];
+  // B: thread owns columns 2*bp, 2*bp+1 and rows bk, bk+8
+  const int bp = tid & 31, bk = tid >> 5;
+  T ra[4];
+  pair_t<T> rb[2];
   auto gload = [&](int k0) {
     const T *pa = A + (int64_t)(m0 + ar) * gb.lda + k0 + ak;
 #pragma unroll
     for (int q = 0; q < 4; q++) ra[q] = pa[q];
 #pragma unroll
-    for (int q = 0; q < 4; q++) rb[q] = load_b<T>(gb, it, k0 + bk + 4 * q, n0 + bn);
+    for (int q = 0; q < 2; q++) rb[q] = load_b2<T>(gb, it, k0 + bk + 8 * q, n0 + 2 * bp);
   };
   typedef typename std::conditional<sizeof(T) == 8, d4, f4>::type acc_t;
   acc_t acc[2][2];
@@ -92,7 +109,10 @@ __global__ void __launch_bounds__(256) k_gemm(GemmBatch gb) {
 #pragma unroll
     for (int q = 0; q < 4; q++) As[ak + q][ar] = ra[q];
 #pragma unroll
-    for (int q = 0; q < 4; q++) Bs[bk + 4 * q][bn] = rb[q];
+    for (int q = 0; q < 2; q++) {
+      Bs[bk + 8 * q][2 * bp] = rb[q].a;
+      Bs[bk + 8 * q][2 * bp + 1] = rb[q].b;
+    }
     __syncthreads();
     if (k0 + BK < gb.K) gload(k0 + BK);
 #pragma unroll
@@ -137,9 +157,11 @@ __device__ __forceinline__ void split_bf16(float v, unsigned short &hi, unsigned
   lo = __builtin_bit_cast(unsigned short, l);
 }
 
+template <int BK>
 __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
-  constexpr int BK = 32;
-  constexpr int KP = BK + 8;  // row stride 80 B: 16-byte aligned, staggers banks
+  constexpr int KP = BK + 8;  // row stride 16-byte aligned, staggers banks
+  constexpr int NA = BK / 32;  // 16-byte A vectors per thread (hi and lo)
+  constexpr int NB = BK / 8;   // B column pairs per thread
   // operands stored k-contiguous so each lane reads 8 bf16 (16 B) at once
   __shared__ __attribute__((aligned(16))) unsigned short Ah[BM][KP], Al[BM][KP];
   __shared__ __attribute__((aligned(16))) unsigned short Bh[BN][KP], Bl[BN][KP];
@@ -148,18 +170,21 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const unsigned short *Ahg = (const unsigned short *)it.A, *Alg = (const unsigned short *)it.A_lo;
-  // A: thread loads row ar, k = ak..ak+7 as one 16-byte vector (hi and lo)
-  const int ar = tid >> 2, ak = (tid & 3) * 8;
-  // B: thread owns column bn and k = bk..bk+7
-  const int bn = tid & 63, bk = (tid >> 6) * 8;
-  us8 rah, ral;
-  float rb[8];
+  // A: thread loads row ar, k = ak .. ak + 8*NA - 1 (hi and lo)
+  const int ar = tid >> 2, ak = (tid & 3) * 8 * NA;
+  // B: thread owns columns 2*bp, 2*bp+1 and k = bk .. bk + NB - 1
+  const int bp = tid & 31, bk = (tid >> 5) * NB;
+  us8 rah[NA], ral[NA];
+  pair_t<float> rb[NB];
   auto gload = [&](int k0) {
-    int64_t o = (int64_t)(m0 + ar) * gb.lda + k0 + ak;
-    rah = *(const us8 *)(Ahg + o);
-    ral = *(const us8 *)(Alg + o);
+    const int64_t o = (int64_t)(m0 + ar) * gb.lda + k0 + ak;
 #pragma unroll
-    for (int q = 0; q < 8; q++) rb[q] = load_b<float>(gb, it, k0 + bk + q, n0 + bn);
+    for (int v = 0; v < NA; v++) {
+      rah[v] = *(const us8 *)(Ahg + o + 8 * v);
+      ral[v] = *(const us8 *)(Alg + o + 8 * v);
+    }
+#pragma unroll
+    for (int q = 0; q < NB; q++) rb[q] = load_b2<float>(gb, it, k0 + bk + q, n0 + 2 * bp);
   };
   f4 acc[2][2], cor[2][2];
 #pragma unroll
@@ -168,41 +193,55 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
     for (int j = 0; j < 2; j++) acc[i][j] = cor[i][j] = (f4){0, 0, 0, 0};
   gload(0);
   for (int k0 = 0; k0 < gb.K; k0 += BK) {
-    *(us8 *)&Ah[ar][ak] = rah;
-    *(us8 *)&Al[ar][ak] = ral;
-    us8 bh, bl;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      unsigned short h, l;
-      split_bf16(rb[q], h, l);
-      bh[q] = h;
-      bl[q] = l;
+    for (int v = 0; v < NA; v++) {
+      *(us8 *)&Ah[ar][ak + 8 * v] = rah[v];
+      *(us8 *)&Al[ar][ak + 8 * v] = ral[v];
     }
-    *(us8 *)&Bh[bn][bk] = bh;
-    *(us8 *)&Bl[bn][bk] = bl;
+#pragma unroll
+    for (int q0 = 0; q0 < NB; q0 += 4) {
+      us4 bh0, bl0, bh1, bl1;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        unsigned short h, l;
+        split_bf16(rb[q0 + q].a, h, l);
+        bh0[q] = h;
+        bl0[q] = l;
+        split_bf16(rb[q0 + q].b, h, l);
+        bh1[q] = h;
+        bl1[q] = l;
+      }
+      *(us4 *)&Bh[2 * bp][bk + q0] = bh0;
+      *(us4 *)&Bl[2 * bp][bk + q0] = bl0;
+      *(us4 *)&Bh[2 * bp + 1][bk + q0] = bh1;
+      *(us4 *)&Bl[2 * bp + 1][bk + q0] = bl1;
+    }
     __syncthreads();
     if (k0 + BK < gb.K) gload(k0 + BK);
-    bf8 ah[2], al[2], bhv[2], blv[2];
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
-      int row = 32 * wr + 16 * i + (lane & 15), k = 8 * (lane >> 4);
-      ah[i] = __builtin_bit_cast(bf8, *(const us8 *)&Ah[row][k]);
-      al[i] = __builtin_bit_cast(bf8, *(const us8 *)&Al[row][k]);
-    }
+    for (int ks = 0; ks < BK; ks += 32) {
+      bf8 ah[2], al[2], bhv[2], blv[2];
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      int col = 32 * wc + 16 * j + (lane & 15), k = 8 * (lane >> 4);
-      bhv[j] = __builtin_bit_cast(bf8, *(const us8 *)&Bh[col][k]);
-      blv[j] = __builtin_bit_cast(bf8, *(const us8 *)&Bl[col][k]);
-    }
-#pragma unroll
-    for (int i = 0; i < 2; i++)
+      for (int i = 0; i < 2; i++) {
+        const int row = 32 * wr + 16 * i + (lane & 15), k = ks + 8 * (lane >> 4);
+        ah[i] = __builtin_bit_cast(bf8, *(const us8 *)&Ah[row][k]);
+        al[i] = __builtin_bit_cast(bf8, *(const us8 *)&Al[row][k]);
+      }
 #pragma unroll
       for (int j = 0; j < 2; j++) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bhv[j], acc[i][j], 0, 0, 0);
-        cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], blv[j], cor[i][j], 0, 0, 0);
-        cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bhv[j], cor[i][j], 0, 0, 0);
+        const int col = 32 * wc + 16 * j + (lane & 15), k = ks + 8 * (lane >> 4);
+        bhv[j] = __builtin_bit_cast(bf8, *(const us8 *)&Bh[col][k]);
+        blv[j] = __builtin_bit_cast(bf8, *(const us8 *)&Bl[col][k]);
       }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bhv[j], acc[i][j], 0, 0, 0);
+          cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], blv[j], cor[i][j], 0, 0, 0);
+          cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bhv[j], cor[i][j], 0, 0, 0);
+        }
+    }
     __syncthreads();
   }
   float *C = (float *)it.C;
@@ -217,6 +256,10 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
         if (row < gb.M && col < gb.N) C[(int64_t)row * gb.ldc + col] = acc[i][j][r] + cor[i][j][r];
       }
 }
+
+#ifndef CHM_BF16_BK
+#define CHM_BF16_BK 32
+#endif
 
 int chm_gemm_launch(const GemmBatch &gb_in, int nitems, int kind, hipStream_t st) {
   if (nitems <= 0) return CHM_OK;
@@ -243,7 +286,7 @@ int chm_gemm_launch(const GemmBatch &gb_in, int nitems, int kind, hipStream_t st
   dim3 grid((gb.N + BN - 1) / BN, (gb.M + BM - 1) / BM, nitems);
   if (kind == GEMM_F64) k_gemm<double><<<grid, 256, 0, st>>>(gb);
   else if (kind == GEMM_F32) k_gemm<float><<<grid, 256, 0, st>>>(gb);
-  else k_gemm_bf16x3<<<grid, 256, 0, st>>>(gb);
+  else k_gemm_bf16x3<CHM_BF16_BK><<<grid, 256, 0, st>>>(gb);
   CHM_CHECK_LAUNCH();
   return CHM_OK;
 }
@@ -284,7 +327,7 @@ extern "C" int chm_pad_table(const double *src, int32_t M, int32_t K, double *d6
 
 // Plain real product for table construction and tests:
 // C[M x N] = A[M x K] @ B[K x N]; A padded as chm_pad_table writes it
-// ([round_up(M,64)][round_up(K,32)]), B and C row-major, contiguous.
+// ([round_up(M,64)][round_up(K,64)]), B and C row-major, contiguous.
 extern "C" int chm_gemm_f64(const double *A, const double *B, double *C, int32_t M, int32_t K,
                             int32_t N, void *stream) {
   GemmBatch gb = {};

@@ -445,7 +445,7 @@ int chm_build_tables_f64(int32_t Nx, int32_t Nr, int32_t M, double dx,
                          size_t ws_bytes, void *stream);
 /* The DHT-type matrices of chm_spectral_t (DHT, DHT_inv, dDp, dDm and their
  * _lo parts) are read in a zero-padded layout: [round_up(Nr-1, 64)] rows of
- * round_up(Nr-1, 32) elements.  chm_dht_table_elems gives its size in
+ * round_up(Nr-1, 64) elements.  chm_dht_table_elems gives its size in
  * elements; chm_pad_table writes it from a row-major [M][K] FP64 matrix as
  * double (d64), float (f32) and/or a split-BF16 pair (bf16_hi + bf16_lo ~=
  * src); any output may be NULL (hi and lo together). */
