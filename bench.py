@@ -26,13 +26,14 @@ mixed vs FP64 at steps 100 and 200) and large (C3, M=2 with a density ramp;
 mixed vs FP64 at steps 100 and 1000) configurations report the same errors
 and charge residuals ("configs").
 
-roofline: the gather + push (the dominant particle kernel), its algorithmic
+roofline: the longer of the two particle kernels of a step (the gather +
+push and the electrons' J deposit, both under "kernels"), its algorithmic
 bytes over its mean duration in the timed steps.  HIP graphs here cannot hold
 event nodes, so the state before the timed steps is saved, the timed steps
 run (graph replay, wall clock), the state is restored and the same steps are
-rerun eagerly with HIP events around the gather in each step; the rerun must
-end bit-identical to the timed run.  traffic: HBM bytes per launch from the
-rocprofv3 counter summary in profiles/.
+rerun eagerly with HIP events around both kernels in each step; the rerun
+must end bit-identical to the timed run.  traffic: HBM bytes per launch from
+the rocprofv3 counter summary in profiles/.
 
 Particle storage: every 20 steps the electrons are permuted into their
 sorted order (PIC_loop(align_every=20), the reference's align gather over all
@@ -359,15 +360,17 @@ def kernel_table(sim, reps, w):
     return rows
 
 
-def pmc_traffic(config, form):
-    """HBM bytes per gather launch measured with rocprofv3 --pmc FETCH_SIZE /
-    WRITE_SIZE (copy-calibrated) on this workload, from profiles/."""
+def pmc_traffic(config, form, kernel="gather"):
+    """HBM bytes per launch of the gather + push or the J deposit measured
+    with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (copy-calibrated) on this
+    workload, from profiles/."""
     if config != "C2":
         return None
-    path = os.path.join(ROOT, "profiles", "gather_push_c2_pmc.md")
+    path = os.path.join(ROOT, "profiles", "particle_kernels_c2_pmc.md")
+    key = "traffic_bytes:" if kernel == "gather" else f"traffic_bytes_{kernel}:"
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes:"):
+            if line.startswith(key):
                 return json.loads(line.split(":", 1)[1]).get(form)
     except OSError:
         return None
@@ -426,60 +429,110 @@ def fingerprint(sim):
         [e[k].clone() for k in ("x", "y", "z", "px", "py", "pz", "g_inv", "sort_indx")]
 
 
-def replay_timed_gather(sim, steps):
+def replay_timed(sim, steps):
     """Runs `steps` PIC steps eagerly (PIC_loop.run without graphs: the
     kernels the graphs replay, in the same order) with a pair of HIP events
-    around the electrons' gather + push in every step, on the stream it is
-    launched on; returns the mean, min and max of its duration (ms).  The
-    host enqueues ahead of the GPU, so each pair brackets the kernel alone."""
+    around the electrons' gather + push and around their J deposit in every
+    step, on the stream they are launched on; returns the mean, min and max
+    of each duration (ms).  The host enqueues ahead of the GPU, so each pair
+    brackets the kernel alone."""
     import torch
     s, e = sim.solver, sim.eons
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(steps)]
-    cur = [0]
-    orig = s._gather_and_push
+    kinds = {"_gather_and_push": "gather", "depose_vector_push": "deposit_J",
+             "depose_vector": "deposit_J"}
+    ev = {"gather": [], "deposit_J": []}
+    flag = {"gather": [], "deposit_J": []}
 
-    pre = [False] * steps
+    def wrap(name):
+        orig = getattr(s, name)
+        kind = kinds[name]
 
-    def timed(parts, flds, **kw):
-        if parts is e:
-            ev[cur[0]][0].record()
-            pre[cur[0]] = bool(kw.get("prepush"))
-        orig(parts, flds, **kw)
-        if parts is e:
-            ev[cur[0]][1].record()
-            cur[0] += 1
+        def timed(parts, *a, **kw):
+            if parts is not e:
+                return orig(parts, *a, **kw)
+            a0, b0 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a0.record()
+            r = orig(parts, *a, **kw)
+            b0.record()
+            ev[kind].append((a0, b0))
+            flag[kind].append(bool(kw.get("prepush")) if kind == "gather"
+                              else name == "depose_vector_push")
+            return r
+        return timed
 
-    s._gather_and_push = timed
+    for name in kinds:
+        setattr(s, name, wrap(name))
     graph, sim.loop.graph = sim.loop.graph, False
     try:
         sim.loop.run(steps)  # eager, the same step sequence as the timed run
     finally:
-        del s._gather_and_push
+        for name in kinds:
+            delattr(s, name)
         sim.loop.graph = graph
     torch.cuda.synchronize()
-    ms = [a.elapsed_time(b) for a, b in ev]
-    return {"mean_ms": float(np.mean(ms)), "min_ms": float(np.min(ms)), "max_ms": float(np.max(ms)),
-            "steps": steps, "prepush_steps": int(sum(pre))}
+    out = {}
+    for kind, pairs in ev.items():
+        ms = [a.elapsed_time(b) for a, b in pairs]
+        out[kind] = {"mean_ms": float(np.mean(ms)), "min_ms": float(np.min(ms)),
+                     "max_ms": float(np.max(ms)), "steps": len(ms),
+                     ("prepush_steps" if kind == "gather" else "push_steps"): int(sum(flag[kind]))}
+    return out
 
 
-def window_roofline(sim, gm, w):
+def _roof(kernel, nbytes, gm, ident, what):
+    t = gm["mean_ms"] * 1e-3
+    achieved = nbytes / t / 1e9
+    return {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "avg_launch_us": t * 1e6, "algorithmic_bytes": nbytes, "algorithmic": what,
+            "measured": f"HIP events around the kernel in each of the {gm['steps']} timed steps "
+                        f"(an eager rerun of the same steps from the saved state), min "
+                        f"{gm['min_ms'] * 1e3:.0f} us, max {gm['max_ms'] * 1e3:.0f} us, rerun "
+                        f"bit-identical to the timed run: {ident}"}
+
+
+def window_roofline(sim, gm, w, ident):
     """Roofline of the gather + push over the timed steps: algorithmic bytes
     over its mean duration in those steps.  Per electron: x, y, z, px, py, pz
     read and px, py, pz, g_inv written (10 words); in a step whose gather also
     applies the next first half push, the pushed x, y, z and the key written
     as well (3 words + 4 bytes)."""
-    t = gm["mean_ms"] * 1e-3
     Np = sim.eons.Args["Np"]
     nbytes = Np * 10 * w + Np * (3 * w + 4) * gm["prepush_steps"] / gm["steps"]
-    achieved = nbytes / t / 1e9
-    return {"kernel": "k_gather_push", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "avg_launch_us": t * 1e6, "algorithmic_bytes": nbytes,
-            "measured": f"HIP events around the kernel in each of the {gm['steps']} timed steps "
-                        f"(an eager rerun of the same steps from the saved state), min "
-                        f"{gm['min_ms'] * 1e3:.0f} us, max {gm['max_ms'] * 1e3:.0f} us, rerun "
-                        f"bit-identical to the timed run: {gm['replay_identical']}"}
+    return _roof("k_gather_push", nbytes, gm, ident,
+                 "10 words per electron, + 3 words + 4 B with the next half push")
+
+
+def deposit_roofline(sim, dm, w, ident):
+    """Roofline of the electrons' J deposit pass (per-cell sums with the
+    fused half push, their node sums and the push of the particles beyond
+    the last full chunk: the three launches of one deposit call) over the
+    timed steps.  Per
+    electron: x, y, z, px, py, pz, g_inv, w read (8 words); with the fused
+    half push, the pushed x, y, z and the key written (3 words + 4 bytes);
+    per launch the J planes written once (3 components, modes 0..M, real
+    words Nr Nx (1 + 2M))."""
+    Np = sim.eons.Args["Np"]
+    g = sim.g
+    planes = 3 * g["Nr"] * g["Nx"] * (1 + 2 * g["M"]) * w
+    nbytes = Np * 8 * w + Np * (3 * w + 4) * dm["push_steps"] / dm["steps"] + planes
+    return _roof("J deposit pass: k_cell_sums<T,1,1,true> + k_node_add<T,1,3> + k_push_keys_tail",
+                 nbytes, dm, ident,
+                 "8 words per electron, + 3 words + 4 B with the half push, + J planes")
+
+
+def dominant_roofline(sim, tm, w, config, form):
+    """The roofline object of the longer of the two particle kernels (gather
+    + push, J deposit) over the timed steps, with the other beside it."""
+    ident = tm["replay_identical"]
+    r = {"gather": window_roofline(sim, tm["gather"], w, ident),
+         "deposit_J": deposit_roofline(sim, tm["deposit_J"], w, ident)}
+    for k in r:
+        r[k]["traffic"] = pmc_traffic(config, form, k)
+    top = max(r, key=lambda k: r[k]["avg_launch_us"])
+    out = dict(r[top])
+    out["kernels"] = r
+    return out
 
 
 def moving_window(sim, events=4, steps_per_event=20):
@@ -818,12 +871,12 @@ def main():
         restore(snap)
         del snap
         sm.loop.it = it0
-        gather_ms[name] = replay_timed_gather(sm, args.steps)
+        gather_ms[name] = replay_timed(sm, args.steps)
         same = fingerprint(sm)
         ident = all(torch.equal(x, y) for x, y in zip(after, same))
         gather_ms[name]["replay_identical"] = ident
         log(f"[bench] {name}: eager replay of the timed steps identical: {ident}; "
-            f"gather {gather_ms[name]}")
+            f"gather {gather_ms[name]['gather']}; J deposit {gather_ms[name]['deposit_J']}")
         del after, same
         sm.steps += args.steps
         log(f"[bench] {name}: {args.steps} steps in {dt:.3f}s: {dt / args.steps * 1e3:.3f} ms/step, "
@@ -836,16 +889,14 @@ def main():
     dt = timing["f64"]
     ms = dt / args.steps * 1e3
     pps = Ne * args.steps / dt
-    roof = window_roofline(fp, gather_ms["f64"], 8)
-    roof["traffic"] = pmc_traffic(args.config, "f64")
+    roof = dominant_roofline(fp, gather_ms["f64"], 8, args.config, "f64")
     mixed = None
     if mx is not None:
         dtm = timing["mixed"]
         mixed = {"value": Ne * args.steps / dtm, "unit": "particle-steps/s",
                  "ms_per_step": dtm / args.steps * 1e3, "pic_steps_per_s": args.steps / dtm,
                  "speedup_vs_f64": dt / dtm, "dtype": "f32 particles/FFT, split-bf16x3 DHT",
-                 "gather_roofline": window_roofline(mx, gather_ms["mixed"], 4)}
-        mixed["gather_roofline"]["traffic"] = pmc_traffic(args.config, "mixed")
+                 "roofline": dominant_roofline(mx, gather_ms["mixed"], 4, args.config, "mixed")}
     kernels = {"f64": kernel_table(fp, args.kernel_reps, 8)}
     if mx is not None:
         kernels["mixed"] = kernel_table(mx, args.kernel_reps, 4)
