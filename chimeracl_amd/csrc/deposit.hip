@@ -44,7 +44,8 @@ struct PlanePtrs {
 // One wave per group of consecutive cells.  Their particles occupy one
 // contiguous range of sorted slots, which the wave stages into LDS in chunks
 // of CAP slots with coalesced loads (sort_indx is close to the identity for a
-// cell-major plasma, so the gathers are nearly coalesced too).  While staging,
+// cell-major plasma, so the gathers are nearly coalesced too); the next
+// chunk's sorted-slot indices are loaded while a chunk is staged.  While staging,
 // each lane computes its particles' CIC weights C_ij (weight included), mode
 // factors ((y + i z)/r)^m and values exactly as the reference kernel does;
 // then one lane per (cell, component, corner) sums its accumulators over the
@@ -63,9 +64,9 @@ template <typename T, int M, int KIND> struct DepCfg {
   static constexpr int NK = VEC ? 3 : 1;              // components
   static constexpr int TPC = 4 * NK;                   // lanes per cell
   static constexpr int CELLS = 64 / TPC;               // cells per wave
-  // staged slots per chunk: smaller chunks (more chunks per wave, fewer
-  // registers and LDS, more waves) measured faster in FP64 at C2
-  static constexpr int CAP = sizeof(T) == 8 ? (VEC ? 96 : 128) : (VEC ? 192 : 256);
+  // staged slots per chunk (measured at C2, with the next chunk's particle
+  // indices loaded ahead)
+  static constexpr int CAP = sizeof(T) == 8 ? (VEC ? 192 : 128) : (VEC ? 192 : 256);
   static constexpr int NV = 4 + 2 * M + (VEC ? 3 : 0); // staged values per particle
 };
 // Lanes walk their own cell's slots, ~ppc apart: one pad element per 32
@@ -106,18 +107,28 @@ __global__ void __launch_bounds__(64)
   T a0 = 0, ar[MM], ai[MM];
 #pragma unroll
   for (int m = 0; m < MM; m++) ar[m] = ai[m] = 0;
+  constexpr int R = (CAP + 63) / 64;
+  // the chunk's particle indices are loaded one chunk ahead
+  uint32_t pix[R];
+#pragma unroll
+  for (int rr = 0; rr < R; rr++) {
+    const uint32_t t = S0 + lane + 64 * rr;
+    pix[rr] = t < S1 ? sort_indx[t] : 0u;
+  }
   for (uint32_t cs = S0; cs < S1; cs += CAP) {
     const uint32_t n = (S1 - cs < CAP) ? S1 - cs : CAP;
     // all loads of the chunk first (one latency), then the weights
-    constexpr int R = (CAP + 63) / 64;
     T lx[R], ly[R], lz[R], lw[R], lu[R][3];
+    uint32_t pnx[R];
 #pragma unroll
     for (int rr = 0; rr < R; rr++) {
       const uint32_t t = lane + 64 * rr;
       lx[rr] = ly[rr] = lz[rr] = lw[rr] = 0;
       lu[rr][0] = lu[rr][1] = lu[rr][2] = 0;
+      const uint32_t tn = cs + CAP + t;
+      pnx[rr] = tn < S1 ? sort_indx[tn] : 0u;
       if (t < n) {
-        const uint32_t p = sort_indx[cs + t];
+        const uint32_t p = pix[rr];
         lx[rr] = x[p];
         ly[rr] = y[p];
         lz[rr] = z[p];
@@ -145,6 +156,8 @@ __global__ void __launch_bounds__(64)
         }
       }
     }
+#pragma unroll
+    for (int rr = 0; rr < R; rr++) pix[rr] = pnx[rr];
 #pragma unroll
     for (int rr = 0; rr < R; rr++) {
       const uint32_t t = lane + 64 * rr;
