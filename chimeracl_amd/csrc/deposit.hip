@@ -765,8 +765,9 @@ __global__ void __launch_bounds__(256)
 
 // The same gather + push visiting sorted slots ip < Np_stay with
 // p = sort_indx[ip] (the reference kernel's own iteration, grid_deposit_m1.cl:
-// 330-350, skipping p >= Np_stay).  A block of GS_NT slots covers a short run
-// of consecutive cells; when those cells lie in one grid row and span at most
+// 330-350, skipping p >= Np_stay).  A block of GS_NT threads takes
+// GS_NT * IT consecutive sorted slots, a short run of consecutive cells; when
+// those cells lie in one grid row and span at most
 // MAXN - 1 cells, the E and B values of the two node rows under them (NV
 // values per node, modes pre-doubled) are staged into LDS once, one node per
 // lane with all of its loads in flight together, and every lane reads its 72
@@ -775,9 +776,11 @@ __global__ void __launch_bounds__(256)
 // bit for bit.
 #define GS_NT 256
 template <typename T, int M> struct GatherCfg {
-  // waves per SIMD the register allocation is held to at M <= 1 (FP64 6,
-  // FP32 8: the fastest C2 steps, no spills); no bound above M = 1
-  static constexpr int WPE = M <= 1 ? (sizeof(T) == 8 ? 6 : 8) : 1;
+  // two sorted slots per thread (their loads in flight together), registers
+  // held to 5 (FP64, M = 1; 4 at M = 0) / 6 (FP32) waves per SIMD, no spills; no
+  // bound above M = 1 (measured on the C2 step)
+  static constexpr int IT = 2;
+  static constexpr int WPE = M <= 1 ? (sizeof(T) == 8 ? (M == 1 ? 5 : 4) : 6) : 1;
   static constexpr int NQ = 1 + 2 * M;
   static constexpr int NV = 6 * NQ;  // E and B, 3 components, all modes
   static constexpr int MN = 20480 / (2 * NV * (int)sizeof(T));
@@ -882,41 +885,57 @@ __global__ void __launch_bounds__(GS_NT) __attribute__((amdgpu_waves_per_eu(Gath
                     T factor_push, int Nx, int Nr, T xmin, T dx_inv, T rmin, T dr_inv,
                     FieldPtrs fp, PrePush<T> pp) {
   typedef GatherCfg<T, M> Cfg;
-  constexpr int NV = Cfg::NV, MAXN = Cfg::MAXN;
+  constexpr int NV = Cfg::NV, MAXN = Cfg::MAXN, IT = Cfg::IT;
   __shared__ T fs[2 * MAXN * NV];
   __shared__ int cab[4];
   const int64_t np_stay = (int64_t)(*np_stay_p);
-  const int64_t s0 = (int64_t)blockIdx.x * GS_NT;
+  const int64_t s0 = (int64_t)blockIdx.x * (GS_NT * IT);
   if (s0 >= np_stay) return;  // uniform over the block
-  const int64_t slot = s0 + threadIdx.x;
-  const bool live = slot < np_stay;
-  const int nlive = (int)((np_stay - s0 < GS_NT) ? np_stay - s0 : GS_NT);
-  uint32_t p = 0;
-  T xp = 0, yp = 0, zp = 0, rp = 0;
-  T u[3] = {0, 0, 0};
-  int ix = 0, ir = 0;
-  if (live) {
-    p = sort_indx[slot];
-    xp = x[p];
-    yp = y[p];
-    zp = z[p];
-    u[0] = px[p];
-    u[1] = py[p];
-    u[2] = pz[p];
-    rp = dsqrt<T>(yp * yp + zp * zp);
-    ix = (int)dfloor<T>((xp - xmin) * dx_inv);
-    ir = (int)dfloor<T>((rp - rmin) * dr_inv);
+  const int nlive = (int)((np_stay - s0 < GS_NT * IT) ? np_stay - s0 : GS_NT * IT);
+  // IT slots per thread (slot s0 + q*GS_NT + tid): all their loads in flight
+  // together
+  uint32_t p[IT];
+  T xp[IT], yp[IT], zp[IT], rp[IT], u[IT][3];
+  int ix[IT], ir[IT];
+#pragma unroll
+  for (int q = 0; q < IT; q++) {
+    const int64_t slot = s0 + q * GS_NT + threadIdx.x;
+    p[q] = slot < np_stay ? sort_indx[slot] : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < IT; q++) {
+    xp[q] = yp[q] = zp[q] = rp[q] = 0;
+    u[q][0] = u[q][1] = u[q][2] = 0;
+    ix[q] = ir[q] = 0;
+    if (s0 + q * GS_NT + threadIdx.x < np_stay) {
+      xp[q] = x[p[q]];
+      yp[q] = y[p[q]];
+      zp[q] = z[p[q]];
+      u[q][0] = px[p[q]];
+      u[q][1] = py[p[q]];
+      u[q][2] = pz[p[q]];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < IT; q++) {
+    if (s0 + q * GS_NT + threadIdx.x < np_stay) {
+      rp[q] = dsqrt<T>(yp[q] * yp[q] + zp[q] * zp[q]);
+      ix[q] = (int)dfloor<T>((xp[q] - xmin) * dx_inv);
+      ir[q] = (int)dfloor<T>((rp[q] - rmin) * dr_inv);
+    }
   }
   // slots below Np_stay hold particles of valid cells, in cell order: the
   // first and the last live slot bound the block's cells
   if (threadIdx.x == 0) {
-    cab[0] = ix;
-    cab[1] = ir;
+    cab[0] = ix[0];
+    cab[1] = ir[0];
   }
-  if ((int)threadIdx.x == nlive - 1) {
-    cab[2] = ix;
-    cab[3] = ir;
-  }
+#pragma unroll
+  for (int q = 0; q < IT; q++)
+    if (q * GS_NT + (int)threadIdx.x == nlive - 1) {
+      cab[2] = ix[q];
+      cab[3] = ir[q];
+    }
   __syncthreads();
   const int ixA = cab[0], irA = cab[1];
   const int ncol = cab[2] - ixA + 2;  // node columns under the block's cells
@@ -927,49 +946,56 @@ __global__ void __launch_bounds__(GS_NT) __attribute__((amdgpu_waves_per_eu(Gath
     stage_node<T, M>(fp, node, fs + threadIdx.x * NV);
   }
   __syncthreads();
-  if (!live) return;
-  if (!((int64_t)(pid ? pid[p] : p) < np_stay)) {
-    // skipped by the reference's Np_stay test: momenta unchanged
-    if (PRE) pre_push<T>(pp, p, xp, yp, zp, u, g_inv[p], Nx, Nr, xmin, dx_inv, rmin, dr_inv);
-    return;
-  }
-  const int64_t igrid = ix + (int64_t)ir * Nx;
-  const int lc = ix - ixA;
-  T dt_2 = (T)0.5 * factor_push;
-  T rp_inv = (T)1. / rp;
-  T sX1 = (xp - xmin) * dx_inv - (T)ix;
-  T sX0 = (T)1.0 - sX1;
-  T sR1 = (rp - rmin) * dr_inv - (T)ir;
-  T sR0 = (T)1.0 - sR1;
-  T gr[M > 0 ? M + 1 : 2], gi[M > 0 ? M + 1 : 2];
-  gr[1] = yp * rp_inv;
-  gi[1] = -zp * rp_inv;
+  const T dt_2 = (T)0.5 * factor_push;
 #pragma unroll
-  for (int m = 2; m <= M; m++) {
-    gr[m] = gr[m - 1] * gr[1] - gi[m - 1] * gi[1];
-    gi[m] = gr[m - 1] * gi[1] + gi[m - 1] * gr[1];
+  for (int q = 0; q < IT; q++) {
+    if (!(s0 + q * GS_NT + threadIdx.x < np_stay)) continue;
+    T(&uq)[3] = u[q];
+    if (!((int64_t)(pid ? pid[p[q]] : p[q]) < np_stay)) {
+      // skipped by the reference's Np_stay test: momenta unchanged
+      if (PRE)
+        pre_push<T>(pp, p[q], xp[q], yp[q], zp[q], uq, g_inv[p[q]], Nx, Nr, xmin, dx_inv, rmin,
+                    dr_inv);
+      continue;
+    }
+    const int64_t igrid = ix[q] + (int64_t)ir[q] * Nx;
+    const int lc = ix[q] - ixA;
+    T rp_inv = (T)1. / rp[q];
+    T sX1 = (xp[q] - xmin) * dx_inv - (T)ix[q];
+    T sX0 = (T)1.0 - sX1;
+    T sR1 = (rp[q] - rmin) * dr_inv - (T)ir[q];
+    T sR0 = (T)1.0 - sR1;
+    T gr[M > 0 ? M + 1 : 2], gi[M > 0 ? M + 1 : 2];
+    gr[1] = yp[q] * rp_inv;
+    gi[1] = -zp[q] * rp_inv;
+#pragma unroll
+    for (int m = 2; m <= M; m++) {
+      gr[m] = gr[m - 1] * gr[1] - gi[m - 1] * gi[1];
+      gi[m] = gr[m - 1] * gi[1] + gi[m - 1] * gr[1];
+    }
+    T C[2][2];
+    C[0][0] = sR0 * sX0;
+    C[0][1] = sR0 * sX1;
+    C[1][0] = sR1 * sX0;
+    C[1][1] = sR1 * sX1;
+    T e_p[3] = {0, 0, 0}, b_p[3] = {0, 0, 0};
+    if (staged) {
+      const T *f0 = fs + lc * NV;
+      gather_sum<T, M, true>([&](int v, int i, int j) { return f0[(i * ncol + j) * NV + v]; }, C,
+                             gr, gi, e_p, b_p);
+    } else {
+      gather_sum<T, M, false>([&](int v, int i, int j) {
+        return field_val<T, M>(fp, v, igrid + j + (int64_t)Nx * i);
+      }, C, gr, gi, e_p, b_p);
+    }
+    const T g_p_inv = boris<T>(uq, e_p, b_p, dt_2);
+    px[p[q]] = uq[0];
+    py[p[q]] = uq[1];
+    pz[p[q]] = uq[2];
+    g_inv[p[q]] = g_p_inv;
+    if (PRE)
+      pre_push<T>(pp, p[q], xp[q], yp[q], zp[q], uq, g_p_inv, Nx, Nr, xmin, dx_inv, rmin, dr_inv);
   }
-  T C[2][2];
-  C[0][0] = sR0 * sX0;
-  C[0][1] = sR0 * sX1;
-  C[1][0] = sR1 * sX0;
-  C[1][1] = sR1 * sX1;
-  T e_p[3] = {0, 0, 0}, b_p[3] = {0, 0, 0};
-  if (staged) {
-    const T *f0 = fs + lc * NV;
-    gather_sum<T, M, true>([&](int v, int i, int j) { return f0[(i * ncol + j) * NV + v]; }, C,
-                           gr, gi, e_p, b_p);
-  } else {
-    gather_sum<T, M, false>([&](int v, int i, int j) {
-      return field_val<T, M>(fp, v, igrid + j + (int64_t)Nx * i);
-    }, C, gr, gi, e_p, b_p);
-  }
-  const T g_p_inv = boris<T>(u, e_p, b_p, dt_2);
-  px[p] = u[0];
-  py[p] = u[1];
-  pz[p] = u[2];
-  g_inv[p] = g_p_inv;
-  if (PRE) pre_push<T>(pp, p, xp, yp, zp, u, g_p_inv, Nx, Nr, xmin, dx_inv, rmin, dr_inv);
 }
 
 template <typename T>
@@ -998,11 +1024,11 @@ static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, 
 #define CASE(MM)                                                                                   \
   case MM:                                                                                         \
     if (pre)                                                                                       \
-      k_gather_sorted<T, MM, true><<<chm_blocks(Np, GS_NT), GS_NT, 0, st>>>(                       \
+      k_gather_sorted<T, MM, true><<<chm_blocks(Np, GS_NT * GatherCfg<T, MM>::IT), GS_NT, 0, st>>>( \
           x, y, z, px, py, pz, g_inv, sort_indx, np_stay, pid, factor_push, g->Nx, g->Nr,          \
           (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp, pp);                             \
     else if (sort_indx)                                                                            \
-      k_gather_sorted<T, MM, false><<<chm_blocks(Np, GS_NT), GS_NT, 0, st>>>(                      \
+      k_gather_sorted<T, MM, false><<<chm_blocks(Np, GS_NT * GatherCfg<T, MM>::IT), GS_NT, 0, st>>>( \
           x, y, z, px, py, pz, g_inv, sort_indx, np_stay, pid, factor_push, g->Nx, g->Nr,          \
           (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp, pp);                             \
     else                                                                                           \
