@@ -771,8 +771,9 @@ def main():
         gather_only(args)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in "
-                          "DESIGN.md but not implemented", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU: particle sharding over several "
+                          "GPUs (an all-reduce of the deposited J and rho) is not implemented",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
     from chimeracl_amd import config as C
