@@ -169,6 +169,29 @@ def test_sort_dense_cells_and_edges(env):
     assert e.Args["Np"] == 0
 
 
+def test_all_particles_outside(env):
+    """Every particle outside the valid cells: Np_stay = 0, the deposits add
+    nothing, the gather + push leaves the momenta unchanged (the reference
+    visits no slot), as in the oracle."""
+    g = env.tiny_grid(Nx=64, Nr=24, M=1)
+    solver, p, s, q = _pair_particles(env, g, 5000, 13, 1)
+    x = np.full(5000, g["Xmax"] + 1.0)
+    p.DataDev["x"].copy_(torch.from_numpy(x))
+    q.data["x"][:] = x
+    p.sort_parts(solver)
+    q.sort_parts(s)
+    assert p.Args["Np_stay"] == 0 == q.Args["Np_stay"]
+    assert np.array_equal(p.DataDev["cell_offset"].cpu().numpy().view(np.uint32), q.data["cell_offset"])
+    solver.depose_charge([p])
+    solver.depose_currents([p])
+    for n in ["rho_m0", "rho_m1"] + [f"J{c}_m{m}" for c in "xyz" for m in (0, 1)]:
+        assert torch.count_nonzero(solver.DataDev[n]) == 0, n
+    u0 = {k: p.host(k) for k in ("px", "py", "pz", "g_inv")}
+    solver.gather_and_push([p])
+    for k, v in u0.items():
+        assert np.array_equal(p.host(k), v), k
+
+
 @pytest.mark.parametrize("case", ["many_tiles", "sparse"])
 def test_sort_large_grid(env, case):
     """A grid of 131,968 bins (18-bit keys, two radix passes): 300,000
