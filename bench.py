@@ -16,8 +16,8 @@ mixed form (FP32 particles and FFT, split-BF16 3-term DHT products); the
 two runs advance in lockstep so that the mixed form's E, B, J and momentum
 errors and both charge-conservation residuals are measured against the FP64
 run at steps 100 and 500.  The FP64 run's error against the CPU FP64 oracle
-is measured after the first 12 steps, with the oracle's throughput on 10 of
-them (a bounded sample) timed on this host's cores.  The differentiable form is
+is measured after the first 32 steps, with the oracle's throughput on 30 of
+them (a bounded sample, ~15 s) timed on this host's cores.  The differentiable form is
 measured on the two differentiable workloads (Nz=1024, Nr=128, 100 steps,
 FP64, gradients against central finite differences; Nz=4096, Nr=256, 400
 steps, mixed, gradients against the FP64 adjoint), with the backward /
@@ -75,7 +75,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--cpu-warmup", type=int, default=2)
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--diag-steps", default="100,500")
