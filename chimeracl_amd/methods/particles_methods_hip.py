@@ -12,7 +12,7 @@ import numpy as np
 import torch
 
 from .. import _lib
-from .generic_methods_hip import GenericMethodsHIP, LazyArgs, ptr
+from .generic_methods_hip import GenericMethodsHIP, ptr
 
 
 def _grid_struct(grid):

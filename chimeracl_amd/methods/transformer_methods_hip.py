@@ -3,7 +3,6 @@ methods/transformer_methods_cl.py and the table construction of
 transformer.py / solver.py, which here runs in FP64 on the GPU)."""
 from __future__ import annotations
 
-import numpy as np
 import torch
 
 from .. import _lib
