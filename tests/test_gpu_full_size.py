@@ -5,7 +5,7 @@ then properties that hold at any size, checked at this one: the sort is the
 stable argsort of the keys with consistent cell offsets, the deposits are
 linear (doubling every weight doubles rho and J bit for bit), and the gather
 + push in a uniform electric field changes every pushed momentum by
-FactorPush * E."""
+FactorPush * E; and the mixed form against the FP64 form after 20 steps."""
 import numpy as np
 import pytest
 
@@ -133,3 +133,48 @@ def test_c2_gather_uniform_field(c2):
     usq = u["px"][pushed] ** 2 + u["py"][pushed] ** 2 + u["pz"][pushed] ** 2
     assert np.max(np.abs(g_inv - 1 / np.sqrt(1 + usq))) < 1e-15
     assert pushed.sum() > 0.99 * e.Args["Np"]
+
+
+def test_c2_mixed_vs_fp64_full_size():
+    """The mixed form (FP32 particles and FFT, split-BF16 DHT) against the
+    FP64 form at full size after 20 steps from the same seeded inputs, each
+    field over all its components and modes: E and B within 1e-4 relative
+    L2 (observed ~3e-6), J and the momenta within 1e-3 (~2e-5, ~1e-5), rho
+    within 1e-2 (~2e-4: electrons and ions nearly cancel, so its relative
+    error measures the cancellation)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import chimeracl_amd as CA
+    from chimeracl_amd import config as C
+    from chimeracl_amd.methods.generic_methods_hip import Communicator
+    from tests import simpair as S
+    g, laser, c = C.grid_config("C2"), C.laser_config("C2"), C.CONFIGS["C2"]
+    arr = C.load_plasma(g, c["ppc"], c["dens"], seed=0, order="cell")
+    runs = {}
+    for prec in ("f64", "mixed"):
+        comm = Communicator(precision=prec)
+        s = CA.Solver(dict(g), comm)
+        CA.add_gausian_pulse(s, dict(laser))
+        eo, io = C.species_config("C2", s.Args)
+        e, i = CA.Particles(dict(eo), comm), CA.Particles(dict(io), comm)
+        e.add_particles(source=C.ParticleSource(arr))
+        i.add_particles(source=C.ParticleSource(arr))
+        loop = CA.PIC_loop(solvers=[s], species=[e, i], graph=True, align_every=20)
+        loop.run(20)
+        torch.cuda.synchronize()
+        runs[prec] = (s, e)
+    (s64, e64), (s32, e32) = runs["f64"], runs["mixed"]
+    errs = {}
+    for fam in ("E", "B", "J", "rho"):
+        names = [n for n in S.real_names(1) if n.startswith(fam)]
+        a = np.concatenate([S.field_host(s64, n)[1:].ravel().astype(np.complex128) for n in names])
+        b = np.concatenate([S.field_host(s32, n)[1:].ravel().astype(np.complex128) for n in names])
+        errs[fam] = S.rel_l2(b, a)
+    pid64, pid32 = e64.DataDev["pid"].cpu().numpy(), e32.DataDev["pid"].cpu().numpy()
+    o64, o32 = np.argsort(pid64), np.argsort(pid32)  # both by the unreordered index
+    for k in ("px", "py", "pz"):
+        errs[k] = S.rel_l2(e32.host(k)[o32], e64.host(k)[o64])
+    tol = {"E": 1e-4, "B": 1e-4, "J": 1e-3, "rho": 1e-2, "px": 1e-3, "py": 1e-3, "pz": 1e-3}
+    print({k: f"{v:.2e}" for k, v in errs.items()})
+    bad = {k: v for k, v in errs.items() if not v < tol[k]}
+    assert not bad, bad
