@@ -41,17 +41,19 @@ struct PlanePtrs {
 };
 
 // ------------------------------------------------------------------ pass 1
-// One wave per group of consecutive cells.  Their particles occupy one
+// One wave per group of CELLS consecutive cells.  Their particles occupy one
 // contiguous range of sorted slots, which the wave stages into LDS in chunks
 // of CAP slots with coalesced loads (sort_indx is close to the identity for a
-// cell-major plasma, so the gathers are nearly coalesced too); the next
-// chunk's sorted-slot indices are loaded while a chunk is staged.  While staging,
-// each lane computes its particles' CIC weights C_ij (weight included), mode
-// factors ((y + i z)/r)^m and values exactly as the reference kernel does;
-// then one lane per (cell, component, corner) sums its accumulators over the
-// cell's particles sequentially, in sorted order.  Every accumulator sees the
-// same operands in the same order as in the reference kernel, so the sums are
-// bit-identical to it.
+// cell-major plasma, so the gathers are nearly coalesced too): all of a
+// chunk's particle values are loaded first, then each lane computes its
+// particles' CIC weights C_ij (weight included), mode factors
+// ((y + i z)/r)^m and values exactly as the reference kernel does; then one
+// lane per (cell, component, corner) sums its accumulators over the cell's
+// particles sequentially, in sorted order.  Every accumulator sees the same
+// operands in the same order as in the reference kernel, so the sums are
+// bit-identical to it.  (Walking several groups per wave, with the next
+// group's loads in flight while a group is summed, measured slower at C2:
+// more groups per wave, slower.)
 //
 // KIND 0: rho (wp = w q); 1: J (wp = w g_inv q, values u); 2: field
 // adjoints of the gather (wp = 1, values in ux/uy/uz, modes weighted 2 e_m).
@@ -64,8 +66,7 @@ template <typename T, int M, int KIND> struct DepCfg {
   static constexpr int NK = VEC ? 3 : 1;              // components
   static constexpr int TPC = 4 * NK;                   // lanes per cell
   static constexpr int CELLS = 64 / TPC;               // cells per wave
-  // staged slots per chunk (measured at C2, with the next chunk's particle
-  // indices loaded ahead)
+  // staged slots per chunk (measured at C2)
   static constexpr int CAP = sizeof(T) == 8 ? (VEC ? 192 : 128) : (VEC ? 192 : 256);
   static constexpr int NV = 4 + 2 * M + (VEC ? 3 : 0); // staged values per particle
 };
@@ -83,72 +84,43 @@ __global__ void __launch_bounds__(64)
                 uint32_t *__restrict__ key_out, T *xo, T *yo, T *zo) {
   typedef DepCfg<T, M, KIND> Cfg;
   constexpr bool VEC = Cfg::VEC;
-  constexpr int CAP = Cfg::CAP, NV = Cfg::NV, TPC = Cfg::TPC;
+  constexpr int CAP = Cfg::CAP, NV = Cfg::NV, TPC = Cfg::TPC, CELLS = Cfg::CELLS;
   constexpr int MM = M > 0 ? M : 1;
   constexpr int VE = 4, VU = 4 + 2 * M;  // first mode-factor / value slot
+  constexpr int R = (CAP + 63) / 64;
+  constexpr int NQ = 1 + 2 * M;
   __shared__ T st[NV][CAP + CAP / 32];
   const int Nx_cell = Nx - 1, Nr_cell = Nr - 1;
   const int64_t ncell = (int64_t)Nx_cell * Nr_cell;
   const int lane = threadIdx.x;
-  const int64_t c0 = (int64_t)blockIdx.x * Cfg::CELLS;
-  const int64_t c1 = (c0 + Cfg::CELLS < ncell) ? c0 + Cfg::CELLS : ncell;
-  const uint32_t S0 = cell_offset[c0], S1 = cell_offset[c1];
   const int task = lane % TPC, k = task >> 2, corner = task & 3;
-  const int64_t c = c0 + lane / TPC;
-  bool act = lane < Cfg::CELLS * TPC && c < c1;
-  uint32_t ip0 = 0, ip1 = 0;
-  if (act) {
-    const int ir = (int)(c / Nx_cell), ix = (int)(c - (int64_t)ir * Nx_cell);
-    act = ix > 0 && ix < Nx_cell - 1 && ir < Nr_cell - 1;
-    ip0 = cell_offset[c];
-    ip1 = cell_offset[c + 1];
-    act = act && ip0 != ip1;
-  }
-  T a0 = 0, ar[MM], ai[MM];
-#pragma unroll
-  for (int m = 0; m < MM; m++) ar[m] = ai[m] = 0;
-  constexpr int R = (CAP + 63) / 64;
-  // the chunk's particle indices are loaded one chunk ahead
-  uint32_t pix[R];
-#pragma unroll
-  for (int rr = 0; rr < R; rr++) {
-    const uint32_t t = S0 + lane + 64 * rr;
-    pix[rr] = t < S1 ? sort_indx[t] : 0u;
-  }
-  for (uint32_t cs = S0; cs < S1; cs += CAP) {
-    const uint32_t n = (S1 - cs < CAP) ? S1 - cs : CAP;
-    // all loads of the chunk first (one latency), then the weights
-    T lx[R], ly[R], lz[R], lw[R], lu[R][3];
-    uint32_t pnx[R];
+  // this wave's cells [c0, c1)
+  const int64_t c0 = (int64_t)blockIdx.x * CELLS;
+  const int64_t c1 = (c0 + CELLS < ncell) ? c0 + CELLS : ncell;
+
+  uint32_t pix[R];  // the chunk's sorted-slot indices
+  auto load_idx = [&](uint32_t (&dst)[R], uint32_t s, uint32_t e) {
 #pragma unroll
     for (int rr = 0; rr < R; rr++) {
       const uint32_t t = lane + 64 * rr;
-      lx[rr] = ly[rr] = lz[rr] = lw[rr] = 0;
+      dst[rr] = (t < CAP && s + t < e) ? sort_indx[s + t] : 0u;
+    }
+  };
+  // particle values of the chunk whose indices are in pix
+  T lx[R], ly[R], lz[R], lw[R], lg[R], lu[R][3];
+  auto load_data = [&](uint32_t n) {
+#pragma unroll
+    for (int rr = 0; rr < R; rr++) {
+      const uint32_t t = lane + 64 * rr;
+      lx[rr] = ly[rr] = lz[rr] = lw[rr] = lg[rr] = 0;
       lu[rr][0] = lu[rr][1] = lu[rr][2] = 0;
-      const uint32_t tn = cs + CAP + t;
-      pnx[rr] = tn < S1 ? sort_indx[tn] : 0u;
       if (t < n) {
         const uint32_t p = pix[rr];
         lx[rr] = x[p];
         ly[rr] = y[p];
         lz[rr] = z[p];
-        if (KIND == 1) {
-          const T gp = g_inv[p];
-          lw[rr] = w[p] * gp * charge;
-          if (PUSH) {
-            const T dt_g = dt * gp;
-            const T xn = lx[rr] + ux[p] * dt_g, yn = ly[rr] + uy[p] * dt_g,
-                    zn = lz[rr] + uz[p] * dt_g;
-            xo[p] = xn;
-            yo[p] = yn;
-            zo[p] = zn;
-            key_out[p] = cell_key<T>(xn, yn, zn, Nx - 1, Nr - 1, xmin, dx_inv, rmin, dr_inv);
-          }
-        } else if (KIND == 0) {
-          lw[rr] = w[p] * charge;
-        } else {
-          lw[rr] = (T)1;
-        }
+        if (KIND != 2) lw[rr] = w[p];
+        if (KIND == 1) lg[rr] = g_inv[p];
         if (VEC) {
           lu[rr][0] = ux[p];
           lu[rr][1] = uy[p];
@@ -156,14 +128,34 @@ __global__ void __launch_bounds__(64)
         }
       }
     }
-#pragma unroll
-    for (int rr = 0; rr < R; rr++) pix[rr] = pnx[rr];
+  };
+  // the chunk's weights, mode factors and values into LDS (and the push)
+  auto stage = [&](uint32_t n) {
 #pragma unroll
     for (int rr = 0; rr < R; rr++) {
       const uint32_t t = lane + 64 * rr;
       if (t >= n) continue;
       const uint32_t q = DEP_SW(t);
-      const T xp = lx[rr], yp = ly[rr], zp = lz[rr], wp = lw[rr];
+      const T xp = lx[rr], yp = ly[rr], zp = lz[rr];
+      T wp;
+      if (KIND == 1) {
+        const T gp = lg[rr];
+        wp = lw[rr] * gp * charge;
+        if (PUSH) {
+          const uint32_t p = pix[rr];
+          const T dt_g = dt * gp;
+          const T xn = xp + lu[rr][0] * dt_g, yn = yp + lu[rr][1] * dt_g,
+                  zn = zp + lu[rr][2] * dt_g;
+          xo[p] = xn;
+          yo[p] = yn;
+          zo[p] = zn;
+          key_out[p] = cell_key<T>(xn, yn, zn, Nx - 1, Nr - 1, xmin, dx_inv, rmin, dr_inv);
+        }
+      } else if (KIND == 0) {
+        wp = lw[rr] * charge;
+      } else {
+        wp = (T)1;
+      }
       const T rp = dsqrt<T>(yp * yp + zp * zp);
       T rp_inv;
       if (VEC) {
@@ -210,9 +202,29 @@ __global__ void __launch_bounds__(64)
         for (int kk = 0; kk < 3; kk++) st[VU + kk][q] = lu[rr][kk];
       }
     }
+  };
+
+  const uint32_t S0 = cell_offset[c0], S1 = cell_offset[c1];
+  const int64_t c = c0 + lane / TPC;
+  bool act = lane < CELLS * TPC && c < c1;
+  uint32_t ip0 = 0, ip1 = 0;
+  if (act) {
+    const int ir = (int)(c / Nx_cell), ix = (int)(c - (int64_t)ir * Nx_cell);
+    act = ix > 0 && ix < Nx_cell - 1 && ir < Nr_cell - 1;
+    ip0 = cell_offset[c];
+    ip1 = cell_offset[c + 1];
+  }
+  T a0 = 0, ar[MM], ai[MM];
+#pragma unroll
+  for (int m = 0; m < MM; m++) ar[m] = ai[m] = 0;
+  for (uint32_t cs = S0; cs < S1; cs += CAP) {
+    const uint32_t n = (S1 - cs < CAP) ? S1 - cs : CAP;
+    load_idx(pix, cs, S1);
+    load_data(n);
+    stage(n);
     __syncthreads();
     if (act) {
-      uint32_t lo = ip0 > cs ? ip0 : cs, hi = ip1 < cs + n ? ip1 : cs + n;
+      const uint32_t lo = ip0 > cs ? ip0 : cs, hi = ip1 < cs + n ? ip1 : cs + n;
 #pragma unroll 4
       for (uint32_t ip = lo; ip < hi; ip++) {
         const uint32_t t = DEP_SW(ip - cs);
@@ -238,9 +250,8 @@ __global__ void __launch_bounds__(64)
     }
     __syncthreads();
   }
-  if (!act) return;
+  if (!act || ip0 == ip1) return;
   // staging layout: part[((k*(1+2M) + q)*4 + corner) * ncell + c]
-  constexpr int NQ = 1 + 2 * M;
   part[((int64_t)(k * NQ) * 4 + corner) * ncell + c] = a0;
 #pragma unroll
   for (int m = 1; m <= M; m++) {
