@@ -599,19 +599,35 @@ def diff_section():
 
     def field_state(sim, a0, R, kv, n):
         """The forward run's final E, B, J planes and electron momenta at the
-        nominal parameters (FP64 copies)."""
+        nominal parameters (FP64 copies), and the charge residual of its last
+        step (rho of step n-1 from a run of n-1 steps; FP64 operators of the
+        run's FP64 twin solver)."""
+        M = sim.solver.Args["M"]
+        with torch.no_grad():
+            sim.forward(a0, R, kv, n - 1)
+        torch.cuda.synchronize()
+        rho_prev = [sim.solver.DataDev[f"rho_fb_m{m}"].to(torch.complex128).clone()
+                    for m in range(M + 1)]
         with torch.no_grad():
             sim.forward(a0, R, kv, n)
         torch.cuda.synchronize()
         d, e = sim.solver.DataDev, sim.eons.DataDev
-        M = sim.solver.Args["M"]
         st = {f: [d[f"{f}{c}_m{m}"][1:].to(torch.complex128 if m else torch.float64).clone()
                   for c in "xyz" for m in range(M + 1)] for f in ("E", "B", "J")}
         st["p"] = [e[k].double().clone() for k in ("px", "py", "pz")]
+        calc = sim._twin_solver()
+        if "divJ_fb_m0" not in calc.DataDev:
+            add_div_scratch(calc)
+        st["charge_residual"] = charge_residual(calc, sim, rho_prev)
         return st
 
     def state_errs(a, ref):
         return {k: _rel(list(zip(a[k], ref[k]))) for k in ("E", "B", "J", "p")}
+
+    def state_report(a, ref, n):
+        return {"step": n, "rel_l2": state_errs(a, ref),
+                "charge_residual": {"fp64": ref["charge_residual"],
+                                    "mixed": a["charge_residual"]}}
 
     gam = lambda outs, rg: A.objective_gamma2(*outs[:3], threshold=5.0, tau=0.25)
     wake = lambda outs, rg: A.objective_wake_energy(list(outs[3:]), rg, ncols=256)
@@ -647,8 +663,7 @@ def diff_section():
     sim, a0, R, kv, n = build("C4", "mixed")
     gmx, res = grads(sim, a0, R, kv, n, gam)
     res["grad_err_vs_fp64_adjoint"] = _grad_errs(gmx, g64)
-    res["fields_vs_fp64_forward"] = {"step": n, "rel_l2": state_errs(field_state(sim, a0, R, kv, n),
-                                                                     ref_state)}
+    res["fields_vs_fp64_forward"] = state_report(field_state(sim, a0, R, kv, n), ref_state, n)
     out["C4_mixed"] = res
     del ref_state
     free(sim)
@@ -669,8 +684,8 @@ def diff_section():
         res["fp64"] = {k: ref[ns][1][k] for k in ("objective", "forward_s", "backward_s",
                                                    "backward_over_forward")}
         out[f"C5_mixed_{ns}steps"] = res
-    out[f"C5_mixed_{n}steps"]["fields_vs_fp64_forward"] = {
-        "step": n, "rel_l2": state_errs(field_state(sim, a0, R, kv, n), ref_state)}
+    out[f"C5_mixed_{n}steps"]["fields_vs_fp64_forward"] = state_report(
+        field_state(sim, a0, R, kv, n), ref_state, n)
     del ref_state
     free(sim)
     return out
