@@ -169,6 +169,46 @@ def test_sort_dense_cells_and_edges(env):
     assert e.Args["Np"] == 0
 
 
+@pytest.mark.parametrize("gi", range(len(GRIDS)))
+def test_deposit_dense_cells_bit_exact(env, gi):
+    """Half of the particles in three adjacent cells (thousands per cell, so
+    a group of cells spans many LDS chunks of the deposit), the rest spread:
+    rho, J and the fused half push with its keys match the oracle bit for
+    bit."""
+    g = env.tiny_grid(**GRIDS[gi])
+    n = 40000
+    solver, p, s, q = _pair_particles(env, g, n, 90 + gi, g["M"])
+    rng = np.random.default_rng(190 + gi)
+    dx = (g["Xmax"] - g["Xmin"]) / (g["Nx"] - 1)
+    dr = g["Rmax"] / (g["Nr"] - 1.5)
+    k = n // 2
+    r = rng.uniform(1.6 * dr, 2.4 * dr, k)
+    th = rng.uniform(0, 2 * np.pi, k)
+    dense = {"x": g["Xmin"] + dx * rng.uniform(10.0, 13.0, k), "y": r * np.sin(th),
+             "z": r * np.cos(th)}
+    for c, v in dense.items():
+        a = p.DataDev[c].clone()
+        a[:k] = torch.as_tensor(v, dtype=a.dtype, device=a.device)
+        p.DataDev[c].copy_(a)
+        q.data[c][:k] = v
+    p.sort_parts(solver)
+    q.sort_parts(s)
+    assert np.array_equal(p.DataDev["cell_offset"].cpu().numpy().view(np.uint32), q.data["cell_offset"])
+    assert np.diff(q.data["cell_offset"].astype(np.int64)).max() > 4000
+    solver.depose_charge([p])
+    s.depose_charge([q])
+    solver.depose_currents(species=[p], push_half=True)
+    s.depose_currents([q])
+    q.push_coords("half")
+    for name in env.real_names(g["M"], flds=("J",)):
+        assert np.array_equal(env.field_host(solver, name), s.f[name]), name
+    for c in ("x", "y", "z"):
+        assert np.array_equal(p.host(c), q.data[c]), c
+    p.sort_parts(solver)
+    q.sort_parts(s)
+    assert np.array_equal(p.DataDev["sort_indx"].cpu().numpy().view(np.uint32), q.data["sort_indx"])
+
+
 def test_all_particles_outside(env):
     """Every particle outside the valid cells: Np_stay = 0, the deposits add
     nothing, the gather + push leaves the momenta unchanged (the reference
