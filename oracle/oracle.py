@@ -177,6 +177,22 @@ def ms_coefficients(w, dt):
     return np.cos(w * dt), np.sin(w * dt) * w, 1.0 / w ** 2
 
 
+def psatd_update(e0, g0, j, n0, n1, c1, c2, c3, dti):
+    """One PSATD step of one spectral component, advance_e_g_m
+    (solver_ms_pic.cl:114-134): the current j and the previous / current
+    density gradients n0 / n1 are scaled by 2 pi (:114-122), then
+      E' = c1 E + c2 c3 (G - j) + c3 (c1 n0 - n1 - (n0 - n1) dt_inv c2 c3)
+      G' = -c2 E + c1 (G - j) + j + c3 (dt_inv (1 - c1) (n0 - n1) - c2 n0)
+    with c1 = cos(w dt), c2 = w sin(w dt), c3 = 1/w^2.  Returns (E', G')."""
+    pi2 = 2 * np.pi
+    j0, n0, n1 = j * pi2, n0 * pi2, n1 * pi2
+    e1 = (c1 * e0 + c2 * c3 * (g0 - j0)
+          + c3 * (c1 * n0 - n1 - (n0 - n1) * dti * c2 * c3))
+    g1 = (-c2 * e0 + c1 * (g0 - j0) + j0
+          + c3 * (dti * (1.0 - c1) * (n0 - n1) - c2 * n0))
+    return e1, g1
+
+
 def damp_profile(N):
     """SolverMethodsCL._init_field_damping, solver_methods_cl.py:32-41."""
     z = np.arange(2 * N)
@@ -598,19 +614,14 @@ class OracleSolver:
     # ---- solver, solver.py:29-39 and solver_ms_pic.cl
     def push_fields(self):
         """advance_e_g_m, solver_ms_pic.cl:57-143, per mode."""
-        f, pi2, dti = self.f, 2 * np.pi, self.Args["dt_inv"]
+        f, dti = self.f, self.Args["dt_inv"]
         for m in range(self.M + 1):
             c1, c2, c3 = self.coef[m]
             for c in self.comps:
                 e0 = f[f"E{c}_fb_m{m}"]
                 g0 = f[f"G{c}_fb_m{m}"]
-                j0 = f[f"J{c}_fb_m{m}"] * pi2
-                n0 = f[f"dN0{c}_fb_m{m}"] * pi2
-                n1 = f[f"dN1{c}_fb_m{m}"] * pi2
-                e1 = (c1 * e0 + c2 * c3 * (g0 - j0)
-                      + c3 * (c1 * n0 - n1 - (n0 - n1) * dti * c2 * c3))
-                g1 = (-c2 * e0 + c1 * (g0 - j0) + j0
-                      + c3 * (dti * (1.0 - c1) * (n0 - n1) - c2 * n0))
+                e1, g1 = psatd_update(e0, g0, f[f"J{c}_fb_m{m}"], f[f"dN0{c}_fb_m{m}"],
+                                      f[f"dN1{c}_fb_m{m}"], c1, c2, c3, dti)
                 e0[:] = e1
                 g0[:] = g1
 
