@@ -65,6 +65,7 @@ _SIGS = {
     "chm_push_x_keys_f64": [P, P, P, P, P, P, P, f64, i64, R(Grid), P, P, sz, P],
     "chm_push_x_keys_f32": [P, P, P, P, P, P, P, f32, i64, R(Grid), P, P, sz, P],
     "chm_sort_from_keys": [i64, R(Grid), P, P, P, P, P, sz, P],
+    "chm_index_sort_f64_noalloc": [P, P, P, i64, R(Grid), P, P, P, P, P],
     "chm_align_elems": [P, i64, P, P, i32, P],
     "chm_dens_profile_f64": [P, P, i64, P, P, P, i32, P],
     "chm_dens_profile_f32": [P, P, i64, P, P, P, i32, P],
@@ -158,7 +159,8 @@ _SIGS = {
 }
 
 ERRORS = {-1: "CHM_EINVAL (argument out of range)", -2: "CHM_EWORKSPACE (workspace too small)",
-          -3: "CHM_EUNSUPPORTED (size not supported)"}
+          -3: "CHM_EUNSUPPORTED (size not supported)",
+          -4: "CHM_ESCRATCH (input larger than the fixed scratch array allows)"}
 
 
 def _load():

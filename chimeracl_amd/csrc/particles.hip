@@ -18,6 +18,8 @@
 // reference's cell_offset; no global atomics, no order fix-up.
 #include "chm_common.h"
 
+#include <mutex>
+
 #define SORT_BS 256
 #define SCAN_ITEMS 8
 #define SCAN_BS 256
@@ -679,6 +681,53 @@ extern "C" int chm_sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_
                                   uint32_t *cell_offset, uint32_t *sort_indx, uint32_t *np_stay,
                                   void *ws, size_t ws_bytes, void *stream) {
   return sort_from_keys(Np, g, indx_in_cell, cell_offset, sort_indx, np_stay, ws, ws_bytes, stream);
+}
+
+// ------------------------------------------------------------------ sort without allocation
+// chm_index_sort_f64_noalloc takes its sort workspace from `scratch`, a fixed
+// array in device memory (one copy per GPU, held until the process ends).
+// In host code `scratch` only names the symbol: its device address on the
+// current GPU comes from hipGetSymbolAddress, looked up once per device on
+// the first call there, and only that address is handed to kernels.  The
+// radix passes overwrite every word they read (tile counts, chunk sums, key
+// and index buffers; the offsets' overflow list is cleared by the first pass
+// or by k_list_clear), so nothing left by an earlier call is read.
+static __device__ __align__(256) unsigned char scratch[CHM_SORT_SCRATCH_BYTES];
+
+#define CHM_MAX_DEVICES 64
+static std::mutex scratch_mu;
+static void *scratch_dev[CHM_MAX_DEVICES];
+
+static int scratch_ptr(void **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= CHM_MAX_DEVICES) return CHM_EUNSUPPORTED;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    if (!p) return CHM_EINVAL;
+    scratch_dev[dev] = p;
+  }
+  *out = scratch_dev[dev];
+  return CHM_OK;
+}
+
+extern "C" int chm_index_sort_f64_noalloc(const double *x, const double *y, const double *z,
+                                          int64_t Np, const chm_grid_t *g, uint32_t *indx_in_cell,
+                                          uint32_t *cell_offset, uint32_t *sort_indx,
+                                          uint32_t *np_stay, void *stream) {
+  if (!g || Np < 0 || Np > 0x7FFFFFF0ll || g->Nx < 4 || g->Nr < 3) return CHM_EINVAL;
+  const int64_t nbins = (int64_t)(g->Nx - 1) * (g->Nr - 1) + 1;
+  const size_t need = sort_layout(Np, nbins, nullptr, nullptr);
+  if (need > (size_t)CHM_SORT_SCRATCH_BYTES) return CHM_ESCRATCH;
+  void *ws = nullptr;
+  int rc = scratch_ptr(&ws);
+  if (rc) return rc;
+  return index_sort<double>(x, y, z, Np, g, indx_in_cell, cell_offset, sort_indx, np_stay, ws, need,
+                            stream);
 }
 
 // ------------------------------------------------------------------ profile

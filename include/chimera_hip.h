@@ -197,6 +197,25 @@ int chm_sort_from_keys(int64_t Np, const chm_grid_t *g,
                        const uint32_t *indx_in_cell, uint32_t *cell_offset,
                        uint32_t *sort_indx, uint32_t *np_stay, void *ws,
                        size_t ws_bytes, void *stream);
+/* chm_index_sort_f64 without a workspace argument and without allocating:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The sort's scratch is a fixed device array of
+ * CHM_SORT_SCRATCH_BYTES per GPU, held until the process ends; inputs larger
+ * than it allows return CHM_ESCRATCH (-4) before anything is launched.  Any
+ * grid of up to 2^22 cells with up to 2^21 particles fits
+ * (chm_sort_workspace_bytes(Np, nbins) <= CHM_SORT_SCRATCH_BYTES is the
+ * exact rule).  Every call on one GPU uses that GPU's one array: calls that
+ * overlap in time on the same GPU (several streams or host threads) race and
+ * must be serialised by the caller.  Outputs are those of
+ * chm_index_sort_f64 bit for bit. */
+#define CHM_SORT_SCRATCH_BYTES (48u << 20)
+#define CHM_ESCRATCH (-4)
+int chm_index_sort_f64_noalloc(const double *x, const double *y,
+                               const double *z, int64_t Np,
+                               const chm_grid_t *g, uint32_t *indx_in_cell,
+                               uint32_t *cell_offset, uint32_t *sort_indx,
+                               uint32_t *np_stay, void *stream);
 /* data_align_dbl / data_align_int (kernels/particles_generic.cl:156-186,
  * ParticleMethodsCL.align_and_damp, methods/particles_methods_cl.py:263-286):
  * dst[i] = src[sort_indx[i]] for i < n; elem_bytes 8 (double) or 4 (float,

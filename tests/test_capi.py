@@ -61,3 +61,19 @@ def test_argument_errors_without_gpu():
     assert rc == -3
     with pytest.raises(RuntimeError):
         _lib.call("chm_push_x_f64", None, None, None, None, None, None, None, 0.1, -5, None)
+
+
+def test_noalloc_sort_rejects_oversize_without_gpu():
+    """The allocation-free sort checks the size against its fixed scratch
+    before it touches the device: 2^22 particles on ~2^22 cells exceed it
+    (CHM_ESCRATCH), up to 2^21 particles on 2^22 cells fit."""
+    from chimeracl_amd import _lib
+    L = _lib.lib
+    g = _lib.Grid(Nx=4096, Nr=1025, M=1)
+    nbins = 4095 * 1024 + 1
+    assert L.chm_sort_workspace_bytes(1 << 21, nbins) <= 48 << 20
+    assert L.chm_sort_workspace_bytes(1 << 22, nbins) > 48 << 20
+    rc = L.chm_index_sort_f64_noalloc(None, None, None, 1 << 22, ctypes.byref(g), None, None,
+                                      None, None, None)
+    assert rc == -4
+
