@@ -66,19 +66,32 @@ HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
 
 
+DEFAULTS = {  # per workload: diagnostics steps, CPU-oracle sample (untimed, timed steps)
+    "C1": {"diag": "100,200", "cpu": (2, 30)},
+    "C2": {"diag": "100,500", "cpu": (2, 20)},
+    "C3": {"diag": "100,1000", "cpu": (1, 3)},
+}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="C2")
+    ap.add_argument("--config", default="C3",
+                    help="headline workload (default C3, the largest one-GPU configuration)")
+    ap.add_argument("--secondary", default="C2",
+                    help="second workload timed the same way ('' to skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=30)
-    ap.add_argument("--cpu-warmup", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="timed CPU-oracle steps (default per workload: C3 3, C2 20, C1 30)")
+    ap.add_argument("--cpu-warmup", type=int, default=None)
     ap.add_argument("--kernel-reps", type=int, default=20)
-    ap.add_argument("--diag-steps", default="100,500")
+    ap.add_argument("--diag-steps", default=None,
+                    help="steps with the mixed-vs-FP64 report (default per workload: C3 "
+                         "100,1000, C2 100,500)")
     ap.add_argument("--align-every", type=int, default=20,
                     help="permute the electrons into their sorted order every K steps (0: never)")
     ap.add_argument("--compare-align", type=int, default=0,
@@ -86,7 +99,8 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (HBM counter calibration)")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip the error report of the small (C1) and large (C3) configurations")
+                    help="skip the error report of the small configuration (C1)")
+    ap.add_argument("--no-window", action="store_true", help="skip the moving-window section")
     ap.add_argument("--gather-only", type=int, default=0, metavar="STEPS",
                     help="only run STEPS eager steps of both forms (no graphs), then three "
                          "gather + push launches of each (for counter collection)")
@@ -149,21 +163,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def load_arrays(cfg_name):
+    """The seeded plasma of a workload (host float64 arrays, cell order)."""
+    from chimeracl_amd import config as C
+    c = C.CONFIGS[cfg_name]
+    t = time.time()
+    arr = C.load_plasma(C.grid_config(cfg_name), c["ppc"], c["dens"], seed=0, order="cell")
+    log(f"[bench] {cfg_name}: loaded {arr['x'].size} particles in {time.time() - t:.1f}s")
+    return arr
+
+
 class Sim:
-    def __init__(self, cfg_name, precision, with_oracle, align_every=0):
+    def __init__(self, cfg_name, precision, with_oracle, align_every=0, arr=None, graph=True):
         import chimeracl_amd as CA
         from chimeracl_amd import config as C
+        from chimeracl_amd.pic_loop import counter_collection_active
         from chimeracl_amd.methods.generic_methods_hip import Communicator
         self.comm = Communicator(precision=precision)
         g = self.g = C.grid_config(cfg_name)
         laser = C.laser_config(cfg_name)
-        c = C.CONFIGS[cfg_name]
         self.solver = CA.Solver(dict(g), self.comm)
         CA.add_gausian_pulse(self.solver, dict(laser))
         eo, io = C.species_config(cfg_name, self.solver.Args)
-        t = time.time()
-        arr = C.load_plasma(g, c["ppc"], c["dens"], seed=0, order="cell")
-        log(f"[bench] {precision}: loaded {arr['x'].size} particles in {time.time() - t:.1f}s")
+        if arr is None:
+            arr = load_arrays(cfg_name)
         self.eons, self.ions = CA.Particles(dict(eo), self.comm), CA.Particles(dict(io), self.comm)
         self.eons.add_particles(source=C.ParticleSource(arr))
         self.ions.add_particles(source=C.ParticleSource(arr))
@@ -171,7 +194,8 @@ class Sim:
         if prof is not None:
             self.eons.dens_profile(*prof)
             self.ions.dens_profile(*prof)
-        self.loop = CA.PIC_loop(solvers=[self.solver], species=[self.eons, self.ions], graph=True,
+        self.loop = CA.PIC_loop(solvers=[self.solver], species=[self.eons, self.ions],
+                                graph=graph and not counter_collection_active(),
                                 align_every=align_every)
         self.steps = 0
         self.orc = None
@@ -361,19 +385,23 @@ def kernel_table(sim, reps, w):
 
 
 def pmc_traffic(config, form, kernel="gather"):
-    """HBM bytes per launch of the gather + push or the J deposit measured
-    with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (copy-calibrated) on this
-    workload, from profiles/."""
-    if config != "C2":
-        return None
-    path = os.path.join(ROOT, "profiles", "particle_kernels_c2_pmc.md")
-    key = "traffic_bytes:" if kernel == "gather" else f"traffic_bytes_{kernel}:"
-    try:
-        for line in open(path):
-            if line.startswith(key):
-                return json.loads(line.split(":", 1)[1]).get(form)
-    except OSError:
-        return None
+    """HBM bytes per launch of one of the roofline kernels (gather, deposit_J,
+    dht) on this workload, measured with rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE (copy-calibrated): read from the counter summaries in
+    profiles/ (files *_<config>_pmc.md, lines `traffic_bytes_<kernel>: {form:
+    bytes}`), or None."""
+    import glob
+    keys = [f"traffic_bytes_{kernel}:"] + (["traffic_bytes:"] if kernel == "gather" else [])
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"*_{config.lower()}_pmc.md"))):
+        try:
+            for line in open(path):
+                for key in keys:
+                    if line.startswith(key):
+                        v = json.loads(line.split(":", 1)[1]).get(form)
+                        if v is not None:
+                            return v
+        except OSError:
+            continue
     return None
 
 
@@ -433,10 +461,13 @@ def replay_timed(sim, steps):
     """Runs `steps` PIC steps eagerly (PIC_loop.run without graphs: the
     kernels the graphs replay, in the same order) with a pair of HIP events
     around the electrons' gather + push and around their J deposit in every
-    step, on the stream they are launched on; returns the mean, min and max
-    of each duration (ms).  The host enqueues ahead of the GPU, so each pair
-    brackets the kernel alone."""
+    step, on the stream they are launched on, and around every DHT product
+    launch (recorded by the library itself on the launch's stream,
+    chm_dht_timing); returns the mean, min and max of each duration (ms).
+    The host enqueues ahead of the GPU, so each pair brackets the kernel
+    alone."""
     import torch
+    from chimeracl_amd import _lib
     s, e = sim.solver, sim.eons
     kinds = {"_gather_and_push": "gather", "depose_vector_push": "deposit_J",
              "depose_vector": "deposit_J"}
@@ -463,19 +494,29 @@ def replay_timed(sim, steps):
     for name in kinds:
         setattr(s, name, wrap(name))
     graph, sim.loop.graph = sim.loop.graph, False
+    _lib.lib.chm_dht_timing(1)
     try:
         sim.loop.run(steps)  # eager, the same step sequence as the timed run
+        torch.cuda.synchronize()
+        nmax = 64 * steps
+        ms, fl = np.zeros(nmax), np.zeros(nmax)
+        n = int(_lib.lib.chm_dht_timing_collect(ms.ctypes.data, fl.ctypes.data, None, nmax))
     finally:
+        _lib.lib.chm_dht_timing(0)
         for name in kinds:
             delattr(s, name)
         sim.loop.graph = graph
-    torch.cuda.synchronize()
     out = {}
     for kind, pairs in ev.items():
-        ms = [a.elapsed_time(b) for a, b in pairs]
-        out[kind] = {"mean_ms": float(np.mean(ms)), "min_ms": float(np.min(ms)),
-                     "max_ms": float(np.max(ms)), "steps": len(ms),
+        t = [a.elapsed_time(b) for a, b in pairs]
+        out[kind] = {"mean_ms": float(np.mean(t)), "min_ms": float(np.min(t)),
+                     "max_ms": float(np.max(t)), "steps": len(t),
                      ("prepush_steps" if kind == "gather" else "push_steps"): int(sum(flag[kind]))}
+    if n > 0:
+        out["dht"] = {"mean_ms": float(ms[:n].mean()), "min_ms": float(ms[:n].min()),
+                      "max_ms": float(ms[:n].max()), "launches": n, "steps": steps,
+                      "launches_per_step": n / steps, "flops_total": float(fl[:n].sum()),
+                      "ms_total": float(ms[:n].sum())}
     return out
 
 
@@ -484,7 +525,8 @@ def _roof(kernel, nbytes, gm, ident, what):
     achieved = nbytes / t / 1e9
     return {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "avg_launch_us": t * 1e6, "algorithmic_bytes": nbytes, "algorithmic": what,
+            "avg_launch_us": t * 1e6, "launches_per_step": 1.0,
+            "ms_per_step": gm["mean_ms"], "algorithmic_bytes": nbytes, "algorithmic": what,
             "measured": f"HIP events around the kernel in each of the {gm['steps']} timed steps "
                         f"(an eager rerun of the same steps from the saved state), min "
                         f"{gm['min_ms'] * 1e3:.0f} us, max {gm['max_ms'] * 1e3:.0f} us, rerun "
@@ -499,7 +541,7 @@ def window_roofline(sim, gm, w, ident):
     as well (3 words + 4 bytes)."""
     Np = sim.eons.Args["Np"]
     nbytes = Np * 10 * w + Np * (3 * w + 4) * gm["prepush_steps"] / gm["steps"]
-    return _roof("k_gather_push", nbytes, gm, ident,
+    return _roof("k_gather_sorted (gather + Boris push)", nbytes, gm, ident,
                  "10 words per electron, + 3 words + 4 B with the next half push")
 
 
@@ -507,29 +549,57 @@ def deposit_roofline(sim, dm, w, ident):
     """Roofline of the electrons' J deposit pass (per-cell sums with the
     fused half push, their node sums and the push of the particles beyond
     the last full chunk: the three launches of one deposit call) over the
-    timed steps.  Per
-    electron: x, y, z, px, py, pz, g_inv, w read (8 words); with the fused
-    half push, the pushed x, y, z and the key written (3 words + 4 bytes);
-    per launch the J planes written once (3 components, modes 0..M, real
-    words Nr Nx (1 + 2M))."""
+    timed steps.  Per electron: x, y, z, px, py, pz, g_inv, w read (8
+    words); with the fused half push, the pushed x, y, z and the key written
+    (3 words + 4 bytes); per launch the J planes written once (3 components,
+    modes 0..M, real words Nr Nx (1 + 2M))."""
     Np = sim.eons.Args["Np"]
     g = sim.g
     planes = 3 * g["Nr"] * g["Nx"] * (1 + 2 * g["M"]) * w
     nbytes = Np * 8 * w + Np * (3 * w + 4) * dm["push_steps"] / dm["steps"] + planes
-    return _roof("J deposit pass: k_cell_sums<T,1,1,true> + k_node_add<T,1,3> + k_push_keys_tail",
+    return _roof("J deposit pass: k_cell_sums + k_node_add + k_push_keys_tail",
                  nbytes, dm, ident,
                  "8 words per electron, + 3 words + 4 B with the half push, + J planes")
 
 
+def dht_roofline(sim, dm, ident):
+    """Roofline of the DHT products (k_gemm / k_gemm_bf16x3) over the timed
+    steps: the algorithmic flops of every launch (2 M K N per product;
+    16 + 32M real products of (Nr-1)^2 Nx per step) over the launches' summed
+    durations, i.e. mean flops per launch over the mean launch duration."""
+    mixed = sim.comm.mixed
+    tf = dm["flops_total"] / (dm["ms_total"] * 1e-3) / 1e12
+    peak = MFMA_PEAK_TFS["f32"] if mixed else MFMA_PEAK_TFS["f64"]
+    out = {"kernel": "k_gemm_bf16x3 (split-BF16 DHT products)" if mixed
+           else "k_gemm<double> (FP64 DHT products)",
+           "bound": "mfma", "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak,
+           "traffic": None, "avg_launch_us": dm["mean_ms"] * 1e3,
+           "launches_per_step": dm["launches_per_step"],
+           "ms_per_step": dm["ms_total"] / dm["steps"],
+           "algorithmic_flops": dm["flops_total"] / dm["launches"],
+           "algorithmic": "2 M K N flops per product (per step (16 + 32 M) 2 (Nr-1)^2 Nx)",
+           "measured": f"HIP events recorded by the library around each of the {dm['launches']} "
+                       f"product launches of the {dm['steps']} timed steps (an eager rerun from "
+                       f"the saved state), min {dm['min_ms'] * 1e3:.0f} us, max "
+                       f"{dm['max_ms'] * 1e3:.0f} us, rerun bit-identical to the timed run: {ident}"}
+    if mixed:
+        out["peak_note"] = "FP32-equivalent flops against the FP32 MFMA peak"
+        out["bf16_mfma_work_frac"] = 3 * tf / MFMA_PEAK_TFS["bf16"]
+    return out
+
+
 def dominant_roofline(sim, tm, w, config, form):
-    """The roofline object of the longer of the two particle kernels (gather
-    + push, J deposit) over the timed steps, with the other beside it."""
+    """The roofline object of the kernel family that takes the most time per
+    step among the DHT products, the gather + push and the J deposit pass,
+    with all three beside it."""
     ident = tm["replay_identical"]
     r = {"gather": window_roofline(sim, tm["gather"], w, ident),
          "deposit_J": deposit_roofline(sim, tm["deposit_J"], w, ident)}
+    if "dht" in tm:
+        r["dht"] = dht_roofline(sim, tm["dht"], ident)
     for k in r:
         r[k]["traffic"] = pmc_traffic(config, form, k)
-    top = max(r, key=lambda k: r[k]["avg_launch_us"])
+    top = max(r, key=lambda k: r[k]["ms_per_step"])
     out = dict(r[top])
     out["kernels"] = r
     return out
@@ -777,26 +847,22 @@ def _grad_errs(g, ref):
 
 
 # ---------------------------------------------------------------- main
-def main():
-    args = parse()
-    if args.calibrate_copy:
-        calibrate_copy()
-        return
-    if args.gather_only:
-        gather_only(args)
-        return
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: particle sharding over several "
-                          "GPUs (an all-reduce of the deposited J and rho) is not implemented",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
+def measure(args, name, diag_at, want_cpu, cpu_sample, with_kernels=True, with_window=True):
+    """One workload in both forms: the CPU-oracle baseline on a bounded sample
+    (and the FP64 run's error against it), the lockstep advance with the
+    mixed-vs-FP64 diagnostics at diag_at, args.steps timed graph-replayed
+    steps per form after args.warmup steps (and after the first diagnostics
+    step), the same steps rerun eagerly with HIP events (rooflines), the
+    per-phase kernel table and the moving-window run."""
     import torch
     from chimeracl_amd import config as C
-    cfg = C.CONFIGS[args.config]
+    cfg = C.CONFIGS[name]
     M = cfg["M"]
-    want_cpu = not args.no_cpu
-    fp = Sim(args.config, "f64", want_cpu, args.align_every)
-    mx = None if args.no_mixed else Sim(args.config, "mixed", False, args.align_every)
+    arr = load_arrays(name)
+    fp = Sim(name, "f64", want_cpu, args.align_every, arr=arr)
+    mx = None if args.no_mixed else Sim(name, "mixed", False, args.align_every, arr=arr)
+    del arr
+    gc.collect()
     sims = [s for s in (fp, mx) if s is not None]
     Ne = fp.eons.Args["Np"]
     add_div_scratch(fp.solver)
@@ -808,33 +874,34 @@ def main():
     cpu = errs = None
     if want_cpu:
         from oracle import oracle as O
+        cw, cs = cpu_sample
         nthreads = O.lib().orc_num_threads()
         s, e, i, oloop = fp.orc
-        for _ in range(args.cpu_warmup):
+        for _ in range(cw):
             oloop.step()
         oloop.Timer = dict.fromkeys(oloop.Timer, 0.0)
         t0 = time.time()
-        for _ in range(args.cpu_steps):
+        for _ in range(cs):
             oloop.step()
         tc = time.time() - t0
-        nst = args.cpu_warmup + args.cpu_steps
+        nst = cw + cs
         for sm in sims:
             sm.step(nst)  # eager first step, captured graph from the second
         torch.cuda.synchronize()
         errs = oracle_errors(fp, M)
         errs["after_steps"] = nst
-        cpu = {"value": Ne * args.cpu_steps / tc, "unit": "particle-steps/s", "cores": nthreads,
-               "kind": "port",
-               "phases_s_per_step": {k: v / args.cpu_steps for k, v in oloop.Timer.items()},
+        cpu = {"value": Ne * cs / tc, "unit": "particle-steps/s", "cores": nthreads,
+               "kind": "port", "pic_steps_per_s": cs / tc,
+               "phases_s_per_step": {k: v / cs for k, v in oloop.Timer.items()},
                "host": host_info(),
-               "sample": f"{args.cpu_steps} full PIC steps (after {args.cpu_warmup} untimed) of the "
-                         f"same workload ({Ne} electrons, {fp.g['Nx']}x{fp.g['Nr']}, M={M}) by the "
-                         f"CPU FP64 oracle (C/OpenMP particle loops, numpy FFT/BLAS), {tc:.1f}s"}
-        log(f"[bench] oracle {args.cpu_steps} steps in {tc:.1f}s on {nthreads} threads; errors {errs}")
+               "sample": f"{cs} full PIC steps (after {cw} untimed) of the same workload ({Ne} "
+                         f"electrons, {fp.g['Nx']}x{fp.g['Nr']}, M={M}) by the CPU FP64 oracle "
+                         f"(C/OpenMP particle loops, numpy FFT/BLAS), {tc:.1f}s"}
+        log(f"[bench] {name} oracle {cs} steps in {tc:.1f}s on {nthreads} threads; errors {errs}")
         fp.orc = None
         del s, e, i, oloop
+        gc.collect()
 
-    diag_at = [int(v) for v in args.diag_steps.split(",") if v]
     diags = {}
 
     def advance_to(n):
@@ -857,7 +924,7 @@ def main():
                                   "energy": energies(mx),
                                   "rel_l2_vs_fp64": mixed_errors(mx, fp, M)}
                 diags[str(nxt)] = d
-                log(f"[bench] step {nxt}: {d}")
+                log(f"[bench] {name} step {nxt}: {d}")
             else:
                 for sm in sims:
                     sm.step(k)
@@ -867,8 +934,8 @@ def main():
         sm.step(args.warmup)
     first_diag = min(diag_at) if diag_at else 0
     advance_to(max(first_diag, fp.steps))
-    timing, gather_ms = {}, {}
-    for name, sm in (("f64", fp), ("mixed", mx)):
+    timing, tm = {}, {}
+    for form, sm in (("f64", fp), ("mixed", mx)):
         if sm is None:
             continue
         snap = snapshot(sm)
@@ -878,84 +945,151 @@ def main():
         sm.loop.run(args.steps)
         torch.cuda.synchronize()
         dt = time.time() - t0
-        timing[name] = dt
-        # the same steps again from the saved state, in a graph with HIP
-        # event nodes around the electron gather: its duration in every
-        # timed step (the step is deterministic, so the replay ends in the
-        # same state, which is checked)
+        timing[form] = dt
+        # the same steps again from the saved state, eagerly, with HIP events
+        # around the gather, the J deposit and every DHT product (the step
+        # is deterministic, so the rerun ends in the same state: checked)
         after = fingerprint(sm)
         restore(snap)
         del snap
         sm.loop.it = it0
-        gather_ms[name] = replay_timed(sm, args.steps)
+        tm[form] = replay_timed(sm, args.steps)
         same = fingerprint(sm)
-        ident = all(torch.equal(x, y) for x, y in zip(after, same))
-        gather_ms[name]["replay_identical"] = ident
-        log(f"[bench] {name}: eager replay of the timed steps identical: {ident}; "
-            f"gather {gather_ms[name]['gather']}; J deposit {gather_ms[name]['deposit_J']}")
+        tm[form]["replay_identical"] = all(torch.equal(x, y) for x, y in zip(after, same))
+        log(f"[bench] {name} {form}: eager rerun identical: {tm[form]['replay_identical']}; "
+            f"gather {tm[form]['gather']}; J deposit {tm[form]['deposit_J']}; "
+            f"DHT {tm[form].get('dht')}")
         del after, same
         sm.steps += args.steps
-        log(f"[bench] {name}: {args.steps} steps in {dt:.3f}s: {dt / args.steps * 1e3:.3f} ms/step, "
-            f"{Ne * args.steps / dt:.4g} particle-steps/s")
+        log(f"[bench] {name} {form}: {args.steps} steps in {dt:.3f}s: "
+            f"{dt / args.steps * 1e3:.3f} ms/step, {Ne * args.steps / dt:.4g} particle-steps/s")
     if mx is not None and mx.steps != fp.steps:
         raise RuntimeError("lockstep broken")
     if diag_at:
         advance_to(max(max(diag_at), fp.steps))
 
     dt = timing["f64"]
-    ms = dt / args.steps * 1e3
-    pps = Ne * args.steps / dt
-    roof = dominant_roofline(fp, gather_ms["f64"], 8, args.config, "f64")
-    mixed = None
+    res = {"workload": f"{name}: " + cfg["desc"], "Nz": cfg["Nx"], "Nr": cfg["Nr"], "modes": M + 1,
+           "ppc": list(cfg["ppc"]), "electrons": Ne, "dt": C.DZ, "align_every": args.align_every,
+           "profile": cfg.get("profile"),
+           "f64": {"value": Ne * args.steps / dt, "unit": "particle-steps/s",
+                   "ms_per_step": dt / args.steps * 1e3, "pic_steps_per_s": args.steps / dt,
+                   "roofline": dominant_roofline(fp, tm["f64"], 8, name, "f64")},
+           "fp64_oracle_error": errs, "cpu_baseline": cpu, "diagnostics": diags,
+           "t_roof_frac": {}}
+    b = roof_bound(fp.g, Ne)
+    res["t_roof_frac"]["f64"] = b["f64_ms"] / res["f64"]["ms_per_step"]
+    res["t_roof_ms"] = b
     if mx is not None:
         dtm = timing["mixed"]
-        mixed = {"value": Ne * args.steps / dtm, "unit": "particle-steps/s",
-                 "ms_per_step": dtm / args.steps * 1e3, "pic_steps_per_s": args.steps / dtm,
-                 "speedup_vs_f64": dt / dtm, "dtype": "f32 particles/FFT, split-bf16x3 DHT",
-                 "roofline": dominant_roofline(mx, gather_ms["mixed"], 4, args.config, "mixed")}
-    kernels = {"f64": kernel_table(fp, args.kernel_reps, 8)}
-    if mx is not None:
-        kernels["mixed"] = kernel_table(mx, args.kernel_reps, 4)
-    window = moving_window(fp)
+        res["mixed"] = {"value": Ne * args.steps / dtm, "unit": "particle-steps/s",
+                        "ms_per_step": dtm / args.steps * 1e3, "pic_steps_per_s": args.steps / dtm,
+                        "speedup_vs_f64": dt / dtm,
+                        "dtype": "f32 particles/FFT, split-bf16x3 DHT",
+                        "roofline": dominant_roofline(mx, tm["mixed"], 4, name, "mixed")}
+        res["t_roof_frac"]["mixed"] = b["mixed_ms"] / res["mixed"]["ms_per_step"]
+    from chimeracl_amd.pic_loop import counter_collection_active
+    if with_kernels and not counter_collection_active():  # the bursts are graph replays
+        res["kernels"] = {"f64": kernel_table(fp, args.kernel_reps, 8)}
+        if mx is not None:
+            res["kernels"]["mixed"] = kernel_table(mx, args.kernel_reps, 4)
+    if with_window:
+        res["moving_window"] = moving_window(fp)
     del fp, mx, sims
     gc.collect()
     torch.cuda.empty_cache()
+    return res
+
+
+def roof_bound(g, Ne):
+    """Lower bound of a step's time: (B_part + B_grid) / 8 TB/s + F_DHT /
+    P_MFMA, with B_part = N_e (21 w + 24), B_grid = 20 r1 + 53 s1 + 3 w S (M+1),
+    F_DHT = (16 + 32 M) 2 (Nr-1)^2 Nx (10 2 (Nr-1)^2 Nx for M = 0)."""
+    Nx, Nr, M = g["Nx"], g["Nr"], g["M"]
+    S, K = (Nr - 1) * Nx, M + 1
+    F = (16 + 32 * M if M > 0 else 10) * 2.0 * (Nr - 1) ** 2 * Nx
+    out = {"F_DHT": F}
+    for form, w, peak in (("f64", 8, MFMA_PEAK_TFS["f64"]), ("mixed", 4, MFMA_PEAK_TFS["f32"])):
+        r1 = Nr * Nx * w * (1 + 2 * M)
+        s1 = 2 * w * S * K
+        bp, bg = Ne * (21 * w + 24), 20 * r1 + 53 * s1 + 3 * w * S * K
+        out[f"{form}_bytes"] = bp + bg
+        out[f"{form}_ms"] = ((bp + bg) / (HBM_PEAK_GBS * 1e9) + F / (peak * 1e12)) * 1e3
+    return out
+
+
+def main():
+    args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    if args.gather_only:
+        gather_only(args)
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: particle sharding over several "
+                          "GPUs (an all-reduce of the deposited J and rho) is not implemented",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch  # noqa: F401
+
+    def diag_for(name):
+        spec = args.diag_steps if (args.diag_steps is not None and name == args.config) \
+            else DEFAULTS.get(name, {}).get("diag", "")
+        return [int(v) for v in spec.split(",") if v]
+
+    def cpu_for(name):
+        cw, cs = DEFAULTS.get(name, {}).get("cpu", (2, 10))
+        if name == args.config:
+            cw = cw if args.cpu_warmup is None else args.cpu_warmup
+            cs = cs if args.cpu_steps is None else args.cpu_steps
+        return cw, cs
+
+    head = measure(args, args.config, diag_for(args.config), not args.no_cpu, cpu_for(args.config),
+                   with_window=not args.no_window)
+    second = None
+    if args.secondary and args.secondary != args.config:
+        second = measure(args, args.secondary, diag_for(args.secondary), False, None,
+                         with_kernels=False, with_window=False)
     aligned = (aligned_section(args, args.compare_align)
                if args.compare_align >= 0 and args.compare_align != args.align_every else None)
     configs = None
-    if not args.no_configs and args.config == "C2":
-        configs = {"C1": config_section("C1", [100, 200], 0 if args.no_cpu else 5, args.align_every),
-                   "C3": config_section("C3", [100, 1000], 0, args.align_every)}
+    if not args.no_configs and "C1" not in (args.config, args.secondary):
+        configs = {"C1": config_section("C1", [100, 200], 0 if args.no_cpu else 5, args.align_every)}
     diff = None
     if not args.no_diff:
         t0 = time.time()
         diff = diff_section()
         log(f"[bench] differentiable section in {time.time() - t0:.1f}s")
 
+    f64 = head["f64"]
+    config = {k: head[k] for k in ("workload", "Nz", "Nr", "modes", "ppc", "electrons", "dt",
+                                    "align_every", "profile")}
     out = {
         "metric": "particle-steps/s",
-        "value": pps,
+        "value": f64["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
-        "pic_steps_per_s": args.steps / dt,
+        "ms_per_step": f64["ms_per_step"],
+        "pic_steps_per_s": f64["pic_steps_per_s"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded uniform plasma, Gaussian laser)",
-        "config": {"workload": f"{args.config}: " + cfg["desc"], "Nz": cfg["Nx"], "Nr": cfg["Nr"],
-                   "modes": M + 1, "ppc": list(cfg["ppc"]), "electrons": Ne, "dt": C.DZ,
-                   "align_every": args.align_every},
-        "fp64_oracle_error": errs,
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "mixed": mixed,
-        "diagnostics": diags,
-        "kernels": kernels,
-        "moving_window": window,
+        "data": "synthetic (seeded plasma with the workload's density profile, Gaussian laser)",
+        "config": config,
+        "fp64_oracle_error": head["fp64_oracle_error"],
+        "roofline": f64["roofline"],
+        "cpu_baseline": head["cpu_baseline"],
+        "mixed": head.get("mixed"),
+        "t_roof_frac": head["t_roof_frac"],
+        "t_roof_ms": head["t_roof_ms"],
+        "diagnostics": head["diagnostics"],
+        "kernels": head.get("kernels"),
+        "moving_window": head.get("moving_window"),
+        "secondary": second,
         "align_compare": aligned,
         "configs": configs,
         "differentiable": diff,

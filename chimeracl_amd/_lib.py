@@ -115,15 +115,17 @@ _SIGS = {
     "chm_tables_workspace_bytes": ([i32, i32, i32], sz),
     "chm_build_tables_f64": [i32, i32, i32, f64, f64, f64, f64, i32, R(TablesOut), P, sz, P],
     "chm_gemm_f64": [P, P, P, i32, i32, i32, P],
+    "chm_dht_timing": [i32],
+    "chm_dht_timing_collect": ([P, P, P, i64], i64),
     "chm_dht_table_elems": ([i32, i32], sz),
     "chm_pad_table": [P, i32, i32, P, P, P, P, P],
     "chm_convert_table": [P, i64, P, P, P, P],
     "chm_laser_pulse_f64": [P, i32, i32, f64, f64, f64, f64, f64, f64, f64, f64, f64, P],
     "chm_laser_spectral_f64": [R(Spectral), P, P, P, f64, P],
     "chm_fill_domain_f64": [P, P, P, P, P, P, P, P, P, P, i32, i64, i32, i32, i32, f64, f64, f64,
-                            f64, ctypes.c_uint64, ctypes.c_uint64, P],
+                            f64, f64, f64, f64, ctypes.c_uint64, ctypes.c_uint64, P],
     "chm_fill_domain_f32": [P, P, P, P, P, P, P, P, P, P, i32, i64, i32, i32, i32, f64, f64, f64,
-                            f64, ctypes.c_uint64, ctypes.c_uint64, P],
+                            f64, f64, f64, f64, ctypes.c_uint64, ctypes.c_uint64, P],
     "chm_phase_tables_f64": [P, i32, f64, P, P, P],
     "chm_fill_beam_f64": [P, P, P, P, P, P, P, P, i64, R(Beam), ctypes.c_uint64, ctypes.c_uint64, P],
     "chm_fill_beam_f32": [P, P, P, P, P, P, P, P, i64, R(Beam), ctypes.c_uint64, ctypes.c_uint64, P],

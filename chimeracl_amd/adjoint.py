@@ -121,7 +121,13 @@ class DiffPIC:
     momenta and of the final real-space Ex planes."""
 
     def __init__(self, grid_cfg, laser_cfg, species_cfgs, plasma, knot_x, comm, ckpt=10,
-                 graph=True):
+                 graph=True, gather_skip="index"):
+        """gather_skip: "index" (default) keeps the reference's gather, which
+        skips particles whose index is >= Np_stay (grid_deposit_m1.cl:367-368:
+        whether a particle is pushed then depends on how many others have
+        left the valid cells, a discontinuity of the objective); "cell" (a
+        deviation from the reference, for gradients) pushes every particle
+        of a valid cell."""
         import chimeracl_amd as CA
         self.comm = comm
         self.ckpt = int(ckpt)
@@ -133,6 +139,12 @@ class DiffPIC:
         from .config import ParticleSource
         self.eons.add_particles(source=ParticleSource(plasma))
         self.ions.add_particles(source=ParticleSource(plasma))
+        if gather_skip not in ("index", "cell"):
+            raise ValueError(f"gather_skip must be 'index' or 'cell', not {gather_skip!r}")
+        self.gather_skip = self.eons.gather_skip = gather_skip
+        # the adjoint gather's activity test compares the index with this
+        # value: the recorded Np_stay, or (rule "cell") the largest uint32
+        self._np_all = torch.full((1,), -1, dtype=torch.int32, device=comm.device)
         dev, real = comm.device, comm.real
         self.base = {k: self.eons.DataDev[k].clone() for k in self.eons._arg_names()}
         self.base_ions = {k: self.ions.DataDev[k].clone() for k in self.ions._arg_names()}
@@ -319,7 +331,8 @@ class DiffPIC:
         fb_arr = (ctypes.c_void_p * 6)(*[ptr(t) for t in self.fb])
         fbp = ctypes.cast(fb_arr, ctypes.c_void_p)
         self._call("chm_gather_push_adj", ptr(x2), ptr(y2), ptr(z2), ptr(ux), ptr(uy), ptr(uz), Np,
-                   ptr(rec["np_stay"]), e.Args["FactorPush"], grid, E_r, B_r, ptr(pb["px"]),
+                   ptr(self._np_all if self.gather_skip == "cell" else rec["np_stay"]),
+                   e.Args["FactorPush"], grid, E_r, B_r, ptr(pb["px"]),
                    ptr(pb["py"]), ptr(pb["pz"]), ptr(pb["g_inv"]), ptr(pb["x"]), ptr(pb["y"]),
                    ptr(pb["z"]), fbp)
         for grp in (self.Er_b, self.Br_b):
@@ -489,7 +502,8 @@ def objective_wake_energy(Ex_planes, rgrid, ncols=256):
     return tot
 
 
-def from_config(name, precision="f64", dht="bf16x3", max_particles=None, ckpt=None, **over):
+def from_config(name, precision="f64", dht="bf16x3", max_particles=None, ckpt=None,
+                gather_skip="index", **over):
     """DiffPIC for one of the workloads of chimeracl_amd.config (C4: Nz=1024,
     Nr=128, 2x2x4 ppc, 32 knots, checkpoints every 10 steps; C5: Nz=4096,
     Nr=256, 2x2x8 ppc, 64 knots, checkpoints every 20 steps).  Returns
@@ -505,5 +519,6 @@ def from_config(name, precision="f64", dht="bf16x3", max_particles=None, ckpt=No
     plasma = C.load_plasma(g, cfg["ppc"], cfg["dens"], seed=0, max_particles=max_particles,
                            order="cell")
     kx, kv = C.density_profile(name, g, **over)
-    sim = DiffPIC(g, las, (eo, io), plasma, kx, comm, ckpt=ckpt or cfg.get("ckpt", 10))
+    sim = DiffPIC(g, las, (eo, io), plasma, kx, comm, ckpt=ckpt or cfg.get("ckpt", 10),
+                  gather_skip=gather_skip)
     return sim, las["a0"], las["R"], np.asarray(kv, dtype=np.float64), cfg["steps"]

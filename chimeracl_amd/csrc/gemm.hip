@@ -20,7 +20,10 @@
 //            B = Bh + Bl, C = Ah*Bh + (Ah*Bl + Al*Bh), the two small terms
 //            accumulated apart and added at the end (mixed, CHM_DHT_BF16X3).
 #include "gemm.h"
+
+#include <mutex>
 #include <type_traits>
+#include <vector>
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -257,9 +260,173 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
       }
 }
 
+// ---------------------------------------------------------------- 128 x 128 tiles
+// The split-BF16 product with a 128 x 128 output tile per block (each wave a
+// 64 x 64 quadrant of 4 x 4 MFMA tiles), for launches with enough tiles to
+// fill the GPU: four times the matrix-core work per LDS round trip and
+// barrier pair, and half the operand loads per flop.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_gemm128_bf16x3(GemmBatch gb) {
+  constexpr int TM = 128, TN = 128, BK = 32, KP = BK + 8;
+  __shared__ __attribute__((aligned(16))) unsigned short Ah[TM][KP], Al[TM][KP];
+  __shared__ __attribute__((aligned(16))) unsigned short Bh[TN][KP], Bl[TN][KP];
+  const GemmItem &it = gb.it[blockIdx.z];
+  const int n0 = blockIdx.x * TN, m0 = blockIdx.y * TM;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const unsigned short *Ahg = (const unsigned short *)it.A, *Alg = (const unsigned short *)it.A_lo;
+  // A: thread loads row ar, k = ak .. ak + 15 (hi and lo), rows beyond the
+  // 64-row padding of the table are not read
+  const int ar = tid >> 1, ak = (tid & 1) * 16;
+  const bool arow = m0 + ar < ((gb.M + 63) & ~63);
+  // B: thread owns columns 2*bp, 2*bp+1 and k = bk .. bk + 7
+  const int bp = tid & 63, bk = (tid >> 6) * 8;
+  us8 rah[2], ral[2];
+  pair_t<float> rb[8];
+  auto gload = [&](int k0) {
+    const int64_t o = (int64_t)(m0 + ar) * gb.lda + k0 + ak;
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      rah[v] = arow ? *(const us8 *)(Ahg + o + 8 * v) : (us8){0, 0, 0, 0, 0, 0, 0, 0};
+      ral[v] = arow ? *(const us8 *)(Alg + o + 8 * v) : (us8){0, 0, 0, 0, 0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) rb[q] = load_b2<float>(gb, it, k0 + bk + q, n0 + 2 * bp);
+  };
+  f4 acc[4][4], cor[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = cor[i][j] = (f4){0, 0, 0, 0};
+  gload(0);
+  for (int k0 = 0; k0 < gb.K; k0 += BK) {
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+      *(us8 *)&Ah[ar][ak + 8 * v] = rah[v];
+      *(us8 *)&Al[ar][ak + 8 * v] = ral[v];
+    }
+    {
+      us8 bh0, bl0, bh1, bl1;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        unsigned short h, l;
+        split_bf16(rb[q].a, h, l);
+        bh0[q] = h;
+        bl0[q] = l;
+        split_bf16(rb[q].b, h, l);
+        bh1[q] = h;
+        bl1[q] = l;
+      }
+      *(us8 *)&Bh[2 * bp][bk] = bh0;
+      *(us8 *)&Bl[2 * bp][bk] = bl0;
+      *(us8 *)&Bh[2 * bp + 1][bk] = bh1;
+      *(us8 *)&Bl[2 * bp + 1][bk] = bl1;
+    }
+    __syncthreads();
+    if (k0 + BK < gb.K) gload(k0 + BK);
+    const int kq = 8 * (lane >> 4);
+    bf8 bhv[4], blv[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int col = 64 * wc + 16 * j + (lane & 15);
+      bhv[j] = __builtin_bit_cast(bf8, *(const us8 *)&Bh[col][kq]);
+      blv[j] = __builtin_bit_cast(bf8, *(const us8 *)&Bl[col][kq]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = 64 * wr + 16 * i + (lane & 15);
+      const bf8 ah = __builtin_bit_cast(bf8, *(const us8 *)&Ah[row][kq]);
+      const bf8 al = __builtin_bit_cast(bf8, *(const us8 *)&Al[row][kq]);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bhv[j], acc[i][j], 0, 0, 0);
+        cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, blv[j], cor[i][j], 0, 0, 0);
+        cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bhv[j], cor[i][j], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  float *C = (float *)it.C;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int row = m0 + 64 * wr + 16 * i + 4 * (lane >> 4) + r;
+        int col = n0 + 64 * wc + 16 * j + (lane & 15);
+        if (row < gb.M && col < gb.N) C[(int64_t)row * gb.ldc + col] = acc[i][j][r] + cor[i][j][r];
+      }
+}
+
 #ifndef CHM_BF16_BK
 #define CHM_BF16_BK 32
 #endif
+
+// ---------------------------------------------------------------- launch timing
+// Measurement aid: while enabled (chm_dht_timing(1)), every product launch
+// on a stream that is not being captured is bracketed by two HIP events
+// recorded on its own stream, kept with the launch's algorithmic flops
+// (2 M K N per item: real FP32-equivalent work for the split-BF16 form).
+struct TimedLaunch {
+  hipEvent_t a, b;
+  double flops;
+  int kind;
+};
+static std::mutex t_mu;
+static bool t_on = false;
+static std::vector<TimedLaunch> t_launches;
+
+static void t_clear() {
+  for (auto &t : t_launches) {
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
+  t_launches.clear();
+}
+
+extern "C" int chm_dht_timing(int32_t enable) {
+  std::lock_guard<std::mutex> lock(t_mu);
+  t_clear();
+  t_on = enable != 0;
+  return CHM_OK;
+}
+
+extern "C" int64_t chm_dht_timing_collect(double *ms, double *flops, int32_t *kind, int64_t max) {
+  std::lock_guard<std::mutex> lock(t_mu);
+  int64_t n = 0;
+  for (auto &t : t_launches) {
+    if (n >= max) break;
+    if (hipEventSynchronize(t.b) != hipSuccess) return -1;
+    float v = 0.f;
+    if (hipEventElapsedTime(&v, t.a, t.b) != hipSuccess) return -1;
+    if (ms) ms[n] = v;
+    if (flops) flops[n] = t.flops;
+    if (kind) kind[n] = t.kind;
+    n++;
+  }
+  t_clear();
+  return n;
+}
+
+static bool t_begin(hipStream_t st, TimedLaunch *t) {
+  std::lock_guard<std::mutex> lock(t_mu);
+  if (!t_on) return false;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+  if (hipEventCreate(&t->a) != hipSuccess) return false;
+  if (hipEventCreate(&t->b) != hipSuccess) {
+    hipEventDestroy(t->a);
+    return false;
+  }
+  hipEventRecord(t->a, st);
+  return true;
+}
+
+static void t_end(hipStream_t st, TimedLaunch *t) {
+  hipEventRecord(t->b, st);
+  std::lock_guard<std::mutex> lock(t_mu);
+  t_launches.push_back(*t);
+}
 
 int chm_gemm_launch(const GemmBatch &gb_in, int nitems, int kind, hipStream_t st) {
   if (nitems <= 0) return CHM_OK;
@@ -283,10 +450,30 @@ int chm_gemm_launch(const GemmBatch &gb_in, int nitems, int kind, hipStream_t st
     }
   }
   const GemmBatch &gb = *pgb;
+  // split-BF16: 128 x 128 tiles when they alone give every CU (256) at least
+  // two blocks (measured at 4096 x 512: 1.41 -> 1.33 ms of products per
+  // step).  The FP64 / FP32 products keep 64 x 64 tiles: the larger tile
+  // measured slower there (3.11 -> 3.65 ms per step at the same size), its
+  // 64 accumulators per lane allow only two waves per SIMD against seven.
+  const int64_t big = (int64_t)((gb.N + 127) / 128) * ((gb.M + 127) / 128) * nitems;
+  const bool use128 = big >= 512;
   dim3 grid((gb.N + BN - 1) / BN, (gb.M + BM - 1) / BM, nitems);
-  if (kind == GEMM_F64) k_gemm<double><<<grid, 256, 0, st>>>(gb);
-  else if (kind == GEMM_F32) k_gemm<float><<<grid, 256, 0, st>>>(gb);
-  else k_gemm_bf16x3<CHM_BF16_BK><<<grid, 256, 0, st>>>(gb);
+  dim3 grid128((gb.N + 127) / 128, (gb.M + 127) / 128, nitems);
+  TimedLaunch tl;
+  const bool timed = t_begin(st, &tl);
+  if (kind == GEMM_F64) {
+    k_gemm<double><<<grid, 256, 0, st>>>(gb);
+  } else if (kind == GEMM_F32) {
+    k_gemm<float><<<grid, 256, 0, st>>>(gb);
+  } else {
+    if (use128) k_gemm128_bf16x3<<<grid128, 256, 0, st>>>(gb);
+    else k_gemm_bf16x3<CHM_BF16_BK><<<grid, 256, 0, st>>>(gb);
+  }
+  if (timed) {
+    tl.flops = 2.0 * gb.M * (double)gb.K * gb.N * nitems;
+    tl.kind = kind;
+    t_end(st, &tl);
+  }
   CHM_CHECK_LAUNCH();
   return CHM_OK;
 }

@@ -825,97 +825,6 @@ __device__ __forceinline__ void det_sincos(double a, double *s_out, double *c_ou
   *c_out = cv;
 }
 
-// fill_grid, particles_generic.cl:33-84: per cell (ix, ir) of the local
-// grids (xgrid [Nx_loc], rgrid), Nppc_th x Nppc_r x Nppc_x particles at the
-// regular sub-positions, the theta set rotated by a per-cell uniform offset
-// in [0, 2 pi) (theta_variator, particles_methods_cl.py:95-98, here from
-// rng_u01(seed, counter0 + cell)); weights r * w0 (:103); momenta p_c and
-// g_inv (:111-140, cold or drifting plasma).
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_fill_domain(T *__restrict__ x, T *__restrict__ y, T *__restrict__ z, T *__restrict__ w,
-                  T *__restrict__ px, T *__restrict__ py, T *__restrict__ pz,
-                  T *__restrict__ g_inv, const double *__restrict__ xgrid,
-                  const double *__restrict__ rgrid, int Nx, int64_t ncells, int npx, int npr,
-                  int npt, double w0, double pxc, double pyc, double pzc, int moving,
-                  uint64_t seed, uint64_t counter0) {
-  const int64_t i_cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i_cell >= ncells) return;
-  const int Nx_cell = Nx - 1;
-  const int ir = (int)(i_cell / Nx_cell), ix = (int)(i_cell - (int64_t)ir * Nx_cell);
-  int64_t ip = i_cell * (int64_t)(npx * npr * npt);
-  const double xmin = xgrid[ix], rmin = rgrid[ir];
-  const double thmin = (2.0 * 3.14159265358979323846) * rng_u01(seed, counter0 + (uint64_t)i_cell);
-  const double Lx = xgrid[ix + 1] - xgrid[ix];
-  const double Lr = rgrid[ir + 1] - rgrid[ir];
-  const double dx = 1. / ((double)npx);
-  const double dr = 1. / ((double)npr);
-  const double dth = 2 * 3.14159265358979323846 / ((double)npt);
-  double g = 1.0;
-  if (pxc != 0.0 || pyc != 0.0 || pzc != 0.0) g = 1. / sqrt(1 + pxc * pxc + pyc * pyc + pzc * pzc);
-  for (int it = 0; it < npt; it++) {
-    const double th = thmin + it * dth;
-    double sin_th, cos_th;
-    det_sincos(th, &sin_th, &cos_th);
-    for (int irr = 0; irr < npr; irr++) {
-      const double rp = rmin + (0.5 + irr) * dr * Lr;
-      const double rp_s = rp * sin_th;
-      const double rp_c = rp * cos_th;
-      for (int ixx = 0; ixx < npx; ixx++) {
-        x[ip] = (T)(xmin + (0.5 + ixx) * dx * Lx);
-        y[ip] = (T)rp_s;
-        z[ip] = (T)rp_c;
-        w[ip] = (T)(rp * w0);
-        if (moving) {
-          px[ip] = (T)pxc;
-          py[ip] = (T)pyc;
-          pz[ip] = (T)pzc;
-          g_inv[ip] = (T)g;
-        }
-        ip += 1;
-      }
-    }
-  }
-}
-
-template <typename T>
-static int fill_domain(T *x, T *y, T *z, T *w, T *px, T *py, T *pz, T *g_inv, const double *xgrid,
-                       const double *rgrid, int32_t Nx_loc, int64_t ncells, int32_t npx,
-                       int32_t npr, int32_t npt, double w0, double pxc, double pyc, double pzc,
-                       uint64_t seed, uint64_t counter0, void *stream) {
-  if (!x || !y || !z || !w || !xgrid || !rgrid || Nx_loc < 2 || ncells < 0 || npx < 1 || npr < 1 ||
-      npt < 1)
-    return CHM_EINVAL;
-  const int moving = px != nullptr;
-  if (moving && (!py || !pz || !g_inv)) return CHM_EINVAL;
-  if (ncells == 0) return CHM_OK;
-  k_fill_domain<T><<<chm_blocks(ncells, 256), 256, 0, (hipStream_t)stream>>>(
-      x, y, z, w, px, py, pz, g_inv, xgrid, rgrid, Nx_loc, ncells, npx, npr, npt, w0, pxc, pyc, pzc,
-      moving, seed, counter0);
-  CHM_CHECK_LAUNCH();
-  return CHM_OK;
-}
-
-extern "C" int chm_fill_domain_f64(double *x, double *y, double *z, double *w, double *px,
-                                   double *py, double *pz, double *g_inv, const double *xgrid,
-                                   const double *rgrid, int32_t Nx_loc, int64_t ncells,
-                                   int32_t npx, int32_t npr, int32_t npt, double w0, double pxc,
-                                   double pyc, double pzc, uint64_t seed, uint64_t counter0,
-                                   void *stream) {
-  return fill_domain<double>(x, y, z, w, px, py, pz, g_inv, xgrid, rgrid, Nx_loc, ncells, npx, npr,
-                             npt, w0, pxc, pyc, pzc, seed, counter0, stream);
-}
-extern "C" int chm_fill_domain_f32(float *x, float *y, float *z, float *w, float *px, float *py,
-                                   float *pz, float *g_inv, const double *xgrid,
-                                   const double *rgrid, int32_t Nx_loc, int64_t ncells,
-                                   int32_t npx, int32_t npr, int32_t npt, double w0, double pxc,
-                                   double pyc, double pzc, uint64_t seed, uint64_t counter0,
-                                   void *stream) {
-  return fill_domain<float>(x, y, z, w, px, py, pz, g_inv, xgrid, rgrid, Nx_loc, ncells, npx, npr,
-                            npt, w0, pxc, pyc, pzc, seed, counter0, stream);
-}
-
-// ------------------------------------------------------------------ beam
 // log from IEEE operations and the exact frexp split, as the CPU oracle
 // evaluates it: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(f),
 // f = (m-1)/(m+1), series to f^19 (|f| <= 0.172); e ln2 with fdlibm's
@@ -953,6 +862,115 @@ __device__ __forceinline__ void det_normal2(uint64_t seed, uint64_t c, double *z
   *z1 = rad * s;
 }
 
+// fill_grid, particles_generic.cl:33-84: per cell (ix, ir) of the local
+// grids (xgrid [Nx_loc], rgrid), Nppc_th x Nppc_r x Nppc_x particles at the
+// regular sub-positions, the theta set rotated by a per-cell uniform offset
+// in [0, 2 pi) (theta_variator, particles_methods_cl.py:95-98, here from
+// rng_u01(seed, counter0 + cell)); weights r * w0 (:103); momenta
+// (:111-147): p_c, plus dp times a standard normal per particle and
+// component when a spread dp is given (_fill_arr_randn(mu = p_c, sigma =
+// dp)); particle ip draws its normals from counters counter_p + 4 ip .. + 2
+// (two Box-Muller pairs, the fourth value unused); g_inv from the momenta.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_fill_domain(T *__restrict__ x, T *__restrict__ y, T *__restrict__ z, T *__restrict__ w,
+                  T *__restrict__ px, T *__restrict__ py, T *__restrict__ pz,
+                  T *__restrict__ g_inv, const double *__restrict__ xgrid,
+                  const double *__restrict__ rgrid, int Nx, int64_t ncells, int npx, int npr,
+                  int npt, double w0, double pxc, double pyc, double pzc, double dpx,
+                  double dpy, double dpz, int moving, uint64_t seed, uint64_t counter0,
+                  uint64_t counter_p) {
+  const int64_t i_cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i_cell >= ncells) return;
+  const int Nx_cell = Nx - 1;
+  const int ir = (int)(i_cell / Nx_cell), ix = (int)(i_cell - (int64_t)ir * Nx_cell);
+  int64_t ip = i_cell * (int64_t)(npx * npr * npt);
+  const double xmin = xgrid[ix], rmin = rgrid[ir];
+  const double thmin = (2.0 * 3.14159265358979323846) * rng_u01(seed, counter0 + (uint64_t)i_cell);
+  const double Lx = xgrid[ix + 1] - xgrid[ix];
+  const double Lr = rgrid[ir + 1] - rgrid[ir];
+  const double dx = 1. / ((double)npx);
+  const double dr = 1. / ((double)npr);
+  const double dth = 2 * 3.14159265358979323846 / ((double)npt);
+  const bool thermal = dpx != 0.0 || dpy != 0.0 || dpz != 0.0;
+  double g = 1.0;
+  if (pxc != 0.0 || pyc != 0.0 || pzc != 0.0) g = 1. / sqrt(1 + pxc * pxc + pyc * pyc + pzc * pzc);
+  for (int it = 0; it < npt; it++) {
+    const double th = thmin + it * dth;
+    double sin_th, cos_th;
+    det_sincos(th, &sin_th, &cos_th);
+    for (int irr = 0; irr < npr; irr++) {
+      const double rp = rmin + (0.5 + irr) * dr * Lr;
+      const double rp_s = rp * sin_th;
+      const double rp_c = rp * cos_th;
+      for (int ixx = 0; ixx < npx; ixx++) {
+        x[ip] = (T)(xmin + (0.5 + ixx) * dx * Lx);
+        y[ip] = (T)rp_s;
+        z[ip] = (T)rp_c;
+        w[ip] = (T)(rp * w0);
+        if (moving) {
+          if (thermal) {
+            double n0, n1, n2, n3;
+            const uint64_t c = counter_p + 4ull * (uint64_t)ip;
+            det_normal2(seed, c, &n0, &n1);
+            det_normal2(seed, c + 2, &n2, &n3);
+            const double ux = pxc + dpx * n0, uy = pyc + dpy * n1, uz = pzc + dpz * n2;
+            px[ip] = (T)ux;
+            py[ip] = (T)uy;
+            pz[ip] = (T)uz;
+            g_inv[ip] = (T)(1. / sqrt(1 + ux * ux + uy * uy + uz * uz));
+          } else {
+            px[ip] = (T)pxc;
+            py[ip] = (T)pyc;
+            pz[ip] = (T)pzc;
+            g_inv[ip] = (T)g;
+          }
+        }
+        ip += 1;
+      }
+    }
+  }
+}
+
+template <typename T>
+static int fill_domain(T *x, T *y, T *z, T *w, T *px, T *py, T *pz, T *g_inv, const double *xgrid,
+                       const double *rgrid, int32_t Nx_loc, int64_t ncells, int32_t npx,
+                       int32_t npr, int32_t npt, double w0, double pxc, double pyc, double pzc,
+                       double dpx, double dpy, double dpz, uint64_t seed, uint64_t counter0,
+                       void *stream) {
+  if (!x || !y || !z || !w || !xgrid || !rgrid || Nx_loc < 2 || ncells < 0 || npx < 1 || npr < 1 ||
+      npt < 1)
+    return CHM_EINVAL;
+  const int moving = px != nullptr;
+  if (moving && (!py || !pz || !g_inv)) return CHM_EINVAL;
+  if (ncells == 0) return CHM_OK;
+  k_fill_domain<T><<<chm_blocks(ncells, 256), 256, 0, (hipStream_t)stream>>>(
+      x, y, z, w, px, py, pz, g_inv, xgrid, rgrid, Nx_loc, ncells, npx, npr, npt, w0, pxc, pyc, pzc,
+      dpx, dpy, dpz, moving, seed, counter0, counter0 + (uint64_t)ncells);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
+extern "C" int chm_fill_domain_f64(double *x, double *y, double *z, double *w, double *px,
+                                   double *py, double *pz, double *g_inv, const double *xgrid,
+                                   const double *rgrid, int32_t Nx_loc, int64_t ncells,
+                                   int32_t npx, int32_t npr, int32_t npt, double w0, double pxc,
+                                   double pyc, double pzc, double dpx, double dpy, double dpz,
+                                   uint64_t seed, uint64_t counter0, void *stream) {
+  return fill_domain<double>(x, y, z, w, px, py, pz, g_inv, xgrid, rgrid, Nx_loc, ncells, npx, npr,
+                             npt, w0, pxc, pyc, pzc, dpx, dpy, dpz, seed, counter0, stream);
+}
+extern "C" int chm_fill_domain_f32(float *x, float *y, float *z, float *w, float *px, float *py,
+                                   float *pz, float *g_inv, const double *xgrid,
+                                   const double *rgrid, int32_t Nx_loc, int64_t ncells,
+                                   int32_t npx, int32_t npr, int32_t npt, double w0, double pxc,
+                                   double pyc, double pzc, double dpx, double dpy, double dpz,
+                                   uint64_t seed, uint64_t counter0, void *stream) {
+  return fill_domain<float>(x, y, z, w, px, py, pz, g_inv, xgrid, rgrid, Nx_loc, ncells, npx, npr,
+                            npt, w0, pxc, pyc, pzc, dpx, dpy, dpz, seed, counter0, stream);
+}
+
+// ------------------------------------------------------------------ beam
 // make_new_beam, particles_methods_cl.py:149-177: Gaussian positions and
 // momenta, equal weights, g_inv from the momenta.  Particle p draws its six
 // normals from counters counter0 + 6p .. + 5.

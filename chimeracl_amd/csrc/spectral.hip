@@ -533,64 +533,65 @@ extern "C" int chm_advance_fields_f32(const chm_spectral_t *sp, const chm_fb_t *
 // One block per (row kr, mode m, component c): optional PSATD of the row,
 // then for E and G: *phase_plus -> IFFT -> (m=0: real part) -> *profile(x)
 // -> FFT -> *phase_minus.  (Solver.damp_fields, solver.py:32-35.)
-// The damping round trip of one spectral row held in registers (element
-// x = threadIdx.x + q*FFT_NT in vv[q]); buf is the row's LDS buffer.  The
-// map is self-adjoint, so the adjoint kernel applies it unchanged.
-template <typename T, int N>
-__device__ __forceinline__ void damp_row(typename C2<T>::t *vv, typename C2<T>::t *buf, int m,
-                                         const T *__restrict__ damp, int nd,
-                                         const T *__restrict__ php, const T *__restrict__ phm,
-                                         const T *__restrict__ tw) {
+// Rows of 4096 run with 512 threads (one radix-8 butterfly per thread and
+// pass, eight row elements per thread), shorter rows with 256.  The E row
+// goes straight into the LDS buffer and only the G row waits in registers,
+// so that a thread holds one row share at a time (at 4096 x FP64 the
+// earlier form that held both in registers spilled to scratch).
+template <int N> struct DampCfg {
+  static constexpr int NT = N >= 4096 ? 512 : FFT_NT;
+  static constexpr int PER = (N + NT - 1) / NT;
+};
+
+// The damping round trip of the row in buf (already multiplied by
+// phase_plus), in place; the map is self-adjoint, so the adjoint kernel
+// applies it unchanged.  Call after a barrier; returns after one.
+template <typename T, int N, int NT>
+__device__ __forceinline__ void damp_lds(typename C2<T>::t *buf, int m, const T *__restrict__ damp,
+                                         int nd, const T *__restrict__ tw) {
   typedef typename C2<T>::t T2;
-  constexpr int PER = (N + FFT_NT - 1) / FFT_NT;
   const T inv = (T)1 / (T)N;
-#pragma unroll
-  for (int q = 0; q < PER; q++) {
-    int x = threadIdx.x + q * FFT_NT;
-    if (x < N) buf[x] = c_mul(vv[q], ((const T2 *)php)[x]);
+  fft_lds<T, N, 1, 1, NT>(buf, (const T2 *)tw);
+  for (int x = threadIdx.x; x < N; x += NT) {
+    T2 v = buf[x];
+    v.x *= inv;
+    v.y *= inv;
+    if (m == 0) v.y = 0;
+    if (x < nd) { v.x *= damp[x]; v.y *= damp[x]; }
+    if (x > N - nd) { v.x *= damp[N - x]; v.y *= damp[N - x]; }
+    buf[x] = v;
   }
   __syncthreads();
-  fft_lds<T, N, 1, 1, FFT_NT>(buf, (const T2 *)tw);
-#pragma unroll
-  for (int q = 0; q < PER; q++) {
-    int x = threadIdx.x + q * FFT_NT;
-    if (x < N) {
-      T2 v = buf[x];
-      v.x *= inv;
-      v.y *= inv;
-      if (m == 0) v.y = 0;
-      if (x < nd) { v.x *= damp[x]; v.y *= damp[x]; }
-      if (x > N - nd) { v.x *= damp[N - x]; v.y *= damp[N - x]; }
-      buf[x] = v;
-    }
-  }
-  __syncthreads();
-  fft_lds<T, N, -1, 1, FFT_NT>(buf, (const T2 *)tw);
-#pragma unroll
-  for (int q = 0; q < PER; q++) {
-    int x = threadIdx.x + q * FFT_NT;
-    if (x < N) vv[q] = c_mul(buf[x], ((const T2 *)phm)[x]);
-  }
-  __syncthreads();
+  fft_lds<T, N, -1, 1, NT>(buf, (const T2 *)tw);
 }
 
+// GREG: the PSATD's new G row waits in registers while E is damped (no
+// extra traffic); otherwise it is stored in place right away and read back
+// for its own round trip (no register array; the reread is of data this
+// block has just written).
+template <int N, bool ADV> struct DampG { static constexpr bool GREG = !ADV; };
+
 template <typename T, int N, bool ADV>
-__global__ void __launch_bounds__(FFT_NT) k_damp(SolvPlanes sv, const T *__restrict__ damp, int nd,
-                                                 const T *__restrict__ php, const T *__restrict__ phm,
-                                                 const T *__restrict__ tw, T dt_inv, int Nk) {
+__global__ void __launch_bounds__(DampCfg<N>::NT) k_damp(SolvPlanes sv, const T *__restrict__ damp,
+                                                         int nd, const T *__restrict__ php,
+                                                         const T *__restrict__ phm,
+                                                         const T *__restrict__ tw, T dt_inv, int Nk) {
   typedef typename C2<T>::t T2;
-  constexpr int PER = (N + FFT_NT - 1) / FFT_NT;
+  constexpr int NT = DampCfg<N>::NT, PER = DampCfg<N>::PER;
+  constexpr bool GREG = DampG<N, ADV>::GREG;
   __shared__ T2 buf[N];
   const int kr = blockIdx.x, m = blockIdx.y, c = blockIdx.z;
   const int64_t rowoff = (int64_t)kr * N;
-  T2 ev[PER], gv[PER];
+  const T2 *pp = (const T2 *)php, *pm = (const T2 *)phm;
+  T2 *E = (T2 *)sv.e[c][m] + rowoff, *G = (T2 *)sv.g[c][m] + rowoff;
+  T2 gv[GREG ? PER : 1];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    int x = threadIdx.x + q * FFT_NT;
+    const int x = threadIdx.x + q * NT;
     if (x < N) {
-      T2 e0 = ((T2 *)sv.e[c][m])[rowoff + x], g0 = ((T2 *)sv.g[c][m])[rowoff + x];
+      T2 e0 = E[x], g0 = G[x];
       if (ADV) {
-        int64_t id = rowoff + x;
+        const int64_t id = rowoff + x;
         T c1 = ((const T *)sv.c1[m])[id], c2 = ((const T *)sv.c2[m])[id], c3 = ((const T *)sv.c3[m])[id];
         T2 j0 = ((const T2 *)sv.j[c][m])[id], n0 = ((const T2 *)sv.n0[c][m])[id], n1 = ((const T2 *)sv.n1[c][m])[id];
         T2 e1, g1;
@@ -599,20 +600,23 @@ __global__ void __launch_bounds__(FFT_NT) k_damp(SolvPlanes sv, const T *__restr
         e0 = e1;
         g0 = g1;
       }
-      ev[q] = e0;
-      gv[q] = g0;
+      buf[x] = c_mul(e0, pp[x]);
+      if constexpr (GREG) gv[q] = g0;
+      else G[x] = g0;
     }
   }
-  damp_row<T, N>(ev, buf, m, damp, nd, php, phm, tw);
-  damp_row<T, N>(gv, buf, m, damp, nd, php, phm, tw);
+  __syncthreads();
+  damp_lds<T, N, NT>(buf, m, damp, nd, tw);
+  for (int x = threadIdx.x; x < N; x += NT) E[x] = c_mul(buf[x], pm[x]);
+  __syncthreads();
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    int x = threadIdx.x + q * FFT_NT;
-    if (x < N) {
-      ((T2 *)sv.e[c][m])[rowoff + x] = ev[q];
-      ((T2 *)sv.g[c][m])[rowoff + x] = gv[q];
-    }
+    const int x = threadIdx.x + q * NT;
+    if (x < N) buf[x] = c_mul(GREG ? gv[q] : G[x], pp[x]);  // G[x]: this thread's own store
   }
+  __syncthreads();
+  damp_lds<T, N, NT>(buf, m, damp, nd, tw);
+  for (int x = threadIdx.x; x < N; x += NT) G[x] = c_mul(buf[x], pm[x]);
 }
 
 template <typename T>
@@ -636,8 +640,10 @@ static int damp_fields(const chm_spectral_t *sp, const chm_fb_t *E, const chm_fb
   switch (Nx) {
 #define CASE(NN)                                                                                       \
   case NN:                                                                                             \
-    if (adv) k_damp<T, NN, true><<<g, FFT_NT, 0, st>>>(sv, dp, sp->ndamp, php, phm, tw, dti, sp->Nr - 1); \
-    else k_damp<T, NN, false><<<g, FFT_NT, 0, st>>>(sv, dp, sp->ndamp, php, phm, tw, dti, sp->Nr - 1); \
+    if (adv) k_damp<T, NN, true><<<g, DampCfg<NN>::NT, 0, st>>>(sv, dp, sp->ndamp, php, phm, tw, dti, \
+                                                                 sp->Nr - 1);                        \
+    else k_damp<T, NN, false><<<g, DampCfg<NN>::NT, 0, st>>>(sv, dp, sp->ndamp, php, phm, tw, dti,   \
+                                                              sp->Nr - 1);                           \
     break;
     CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096)
 #undef CASE
@@ -909,19 +915,20 @@ __device__ __forceinline__ void psatd_adj_store(const SolvAdj &sa, int c, int m,
 }
 
 template <typename T, int N>
-__global__ void __launch_bounds__(FFT_NT) k_damp_adj(SolvAdj sa, const T *__restrict__ damp, int nd,
-                                                     const T *__restrict__ php,
-                                                     const T *__restrict__ phm,
-                                                     const T *__restrict__ tw, T dt_inv) {
+__global__ void __launch_bounds__(DampCfg<N>::NT) k_damp_adj(SolvAdj sa, const T *__restrict__ damp,
+                                                             int nd, const T *__restrict__ php,
+                                                             const T *__restrict__ phm,
+                                                             const T *__restrict__ tw, T dt_inv) {
   typedef typename C2<T>::t T2;
-  constexpr int PER = (N + FFT_NT - 1) / FFT_NT;
+  constexpr int NT = DampCfg<N>::NT, PER = DampCfg<N>::PER;
   __shared__ T2 buf[N];
   const int kr = blockIdx.x, m = blockIdx.y, c = blockIdx.z;
   const int64_t rowoff = (int64_t)kr * N;
+  const T2 *pp = (const T2 *)php, *pm = (const T2 *)phm;
   T2 ev[PER], gv[PER];
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    int x = threadIdx.x + q * FFT_NT;
+    const int x = threadIdx.x + q * NT;
     if (x < N) {
       T2 e = ((T2 *)sa.e[c][m])[rowoff + x];
       if (sa.eadd[c][m]) {
@@ -929,16 +936,29 @@ __global__ void __launch_bounds__(FFT_NT) k_damp_adj(SolvAdj sa, const T *__rest
         e.x += a.x;
         e.y += a.y;
       }
-      ev[q] = e;
+      buf[x] = c_mul(e, pp[x]);
       gv[q] = ((T2 *)sa.g[c][m])[rowoff + x];
     }
   }
-  damp_row<T, N>(ev, buf, m, damp, nd, php, phm, tw);
-  damp_row<T, N>(gv, buf, m, damp, nd, php, phm, tw);
+  __syncthreads();
+  damp_lds<T, N, NT>(buf, m, damp, nd, tw);
 #pragma unroll
   for (int q = 0; q < PER; q++) {
-    int x = threadIdx.x + q * FFT_NT;
-    if (x < N) psatd_adj_store<T>(sa, c, m, rowoff + x, ev[q], gv[q], dt_inv);
+    const int x = threadIdx.x + q * NT;
+    if (x < N) ev[q] = c_mul(buf[x], pm[x]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int x = threadIdx.x + q * NT;
+    if (x < N) buf[x] = c_mul(gv[q], pp[x]);
+  }
+  __syncthreads();
+  damp_lds<T, N, NT>(buf, m, damp, nd, tw);
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int x = threadIdx.x + q * NT;
+    if (x < N) psatd_adj_store<T>(sa, c, m, rowoff + x, ev[q], c_mul(buf[x], pm[x]), dt_inv);
   }
 }
 
@@ -998,7 +1018,7 @@ static int damp_fields_adj(const chm_spectral_t *sp, const chm_fb_t *E, const ch
   switch (Nx) {
 #define CASE(NN)                                                                   \
   case NN:                                                                         \
-    k_damp_adj<T, NN><<<g, FFT_NT, 0, st>>>(sa, dp, sp->ndamp, php, phm, tw, dti); \
+    k_damp_adj<T, NN><<<g, DampCfg<NN>::NT, 0, st>>>(sa, dp, sp->ndamp, php, phm, tw, dti); \
     break;
     CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096)
 #undef CASE

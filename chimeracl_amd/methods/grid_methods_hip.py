@@ -43,9 +43,15 @@ class GridMethodsHIP(GenericMethodsHIP):
             g = self._pcache["grid"] = _grid_struct(self)
         return g
 
-    def _frame_moved(self):
-        """After Frame.shift_grids: rebuild the cached geometry (and, for a
-        solver, the transform phases of the new window origin)."""
+    def translate_window(self, dist):
+        """Moves the longitudinal axis by dist (Frame.shift_grids,
+        frame.py:22-30: Xmin, Xmax and Xgrid, host and device copies), then
+        rebuilds the cached geometry and, for a solver, the transform phases
+        e^(-/+ i kx Xmin) of the new origin."""
+        for key in ("Xmin", "Xmax", "Xgrid"):
+            self.Args[key] = self.Args[key] + dist
+            if key in self.DataDev:
+                self.DataDev[key] += dist
         self._pcache.pop("grid", None)
         if hasattr(self, "_refresh_phases"):
             self._refresh_phases()
@@ -165,14 +171,15 @@ class GridMethodsHIP(GenericMethodsHIP):
             self.call("chm_gather_push_pre", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]),
                       ptr(d["py"]), ptr(d["pz"]), ptr(d["g_inv"]), parts.Args["Np"],
                       ptr(d["sort_indx"]), ptr(d["cell_offset"]), ptr(d["Np_stay"]),
-                      ptr(d.get("pid")), parts.Args["FactorPush"], parts.Args["dt_2"],
+                      ptr(parts.gather_skip_ids()), parts.Args["FactorPush"], parts.Args["dt_2"],
                       ptr(d["xh"]), ptr(d["yh"]), ptr(d["zh"]), ptr(d["indx_in_cell"]),
                       self._grid(), E, B)
             return
         args = (ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]), ptr(d["py"]), ptr(d["pz"]),
                 ptr(d["g_inv"]), parts.Args["Np"], ptr(d["sort_indx"]), ptr(d["Np_stay"]))
-        if "pid" in d:  # storage reordered: the skip uses the unreordered index
-            self.call("chm_gather_push_ids", *args, ptr(d["pid"]), parts.Args["FactorPush"],
+        ids = parts.gather_skip_ids()
+        if ids is not None:  # reordered storage (or the "cell" skip rule)
+            self.call("chm_gather_push_ids", *args, ptr(ids), parts.Args["FactorPush"],
                       self._grid(), E, B)
         else:
             self.call("chm_gather_push", *args, parts.Args["FactorPush"], self._grid(), E, B)

@@ -61,8 +61,12 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         parts_in (Xmin, Xmax, Rmin, Rmax) into DataDev['<c>_new'], filled on the
         GPU (fill_grid, particles_generic.cl:33-84).  The per-cell theta offsets
         come from a counter-based generator (Args 'seed', advancing
-        Args['rng_counter']) instead of PyOpenCL's.  Cold or drifting plasma
-        (px_c, py_c, pz_c); a thermal spread (dpx, ...) is not supported."""
+        Args['rng_counter']) instead of PyOpenCL's.  Momenta p_c (px_c, ...)
+        plus, for a component with a spread (dpx, ...), dp times a standard
+        normal per particle (_fill_arr_randn(mu=p_c, sigma=dp),
+        particles_methods_cl.py:114-147), drawn by Box-Muller from the same
+        generator; as in the reference, missing p_c / dp keys are set to 0 in
+        parts_in."""
         a = self.Args
         xmin, xmax, rmin, rmax = (parts_in[k] for k in ("Xmin", "Xmax", "Rmin", "Rmax"))
         Nx_loc = int(np.ceil((xmax - xmin) / a["dx"]) + 1)
@@ -76,14 +80,11 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         dev, real = self.comm.device, self.comm.real
         new = {k: torch.empty(Np, dtype=real, device=dev) for k in ("x", "y", "z", "w")}
         moving = "Immobile" not in a
-        pc = [0.0, 0.0, 0.0]
+        pc, dp = [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]
         if moving:
             for i, k in enumerate(("px", "py", "pz")):
-                if parts_in.get("d" + k, 0):
-                    raise NotImplementedError(
-                        "thermal momentum spreads (d%s) need a normal generator; only cold or "
-                        "drifting plasma is filled on the GPU" % k)
-                pc[i] = float(parts_in.get(k + "_c", 0.0))
+                pc[i] = float(parts_in.setdefault(k + "_c", 0))
+                dp[i] = float(parts_in.setdefault("d" + k, 0))
             for k in ("px", "py", "pz", "g_inv"):
                 new[k] = torch.empty(Np, dtype=real, device=dev)
         grids = [torch.from_numpy(v).to(dev) for v in (xg, rg)]
@@ -91,8 +92,9 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         p = lambda k: ptr(new[k]) if k in new else None
         self.call("chm_fill_domain", p("x"), p("y"), p("z"), p("w"), p("px"), p("py"), p("pz"),
                   p("g_inv"), ptr(grids[0]), ptr(grids[1]), Nx_loc, ncells, npx, npr, npt,
-                  float(a["w0"]), pc[0], pc[1], pc[2], int(a.get("seed", 0)), counter)
-        a["rng_counter"] = counter + ncells
+                  float(a["w0"]), pc[0], pc[1], pc[2], dp[0], dp[1], dp[2], int(a.get("seed", 0)),
+                  counter)
+        a["rng_counter"] = counter + ncells + (4 * Np if any(dp) else 0)
         for k, t in new.items():
             self.DataDev[k + "_new"] = t
         if density_profiles is not None:
@@ -287,6 +289,22 @@ class ParticleMethodsHIP(GenericMethodsHIP):
     def free_added(self):
         for arg in self._arg_names():
             self.DataDev.pop(arg + "_new", None)
+
+    def gather_skip_ids(self):
+        """The ids the gather compares with Np_stay (the reference's skip of
+        particles with index >= Np_stay, grid_deposit_m1.cl:367-368): the
+        unreordered index DataDev['pid'] if the storage was reordered, else
+        None (the storage index itself).  With ``self.gather_skip = "cell"``
+        (an option of the differentiable form, off by default) every visited
+        particle -- every particle in a valid cell -- is pushed: the ids are
+        all zero, so the index test never skips one."""
+        if getattr(self, "gather_skip", "index") != "cell":
+            return self.DataDev.get("pid")
+        z = getattr(self, "_zero_ids", None)
+        if z is None or z.numel() != self.Args["Np"]:
+            z = self._zero_ids = torch.zeros(max(self.Args["Np"], 1), dtype=torch.int32,
+                                             device=self.comm.device)
+        return z
 
     def host(self, comp):
         """Host copy of a particle component (float64 numpy)."""

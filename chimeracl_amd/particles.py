@@ -8,8 +8,14 @@ import numpy as np
 from .methods.generic_methods_hip import LazyArgs, default_comm
 from .methods.particles_methods_hip import ParticleMethodsHIP
 
-# scipy.constants values used by the reference's w2pC (particles.py:96)
+# weight -> charge in pC per micron of wavelength (particles.py:96), from the
+# CODATA 2018 values of scipy.constants: 4 pi^2 m_e c^2 eps0 1e6 / e
 _M_E, _C, _E, _EPS0 = 9.1093837015e-31, 299792458.0, 1.602176634e-19, 8.8541878128e-12
+_W2PC = 4 * np.pi ** 2 * _M_E * _C ** 2 * _EPS0 * 1e6 / _E
+
+# configuration defaults of a species (particles.py:55-70)
+_ARG_DEFAULTS = (("dt", 1.0), ("dx", 1.0), ("dr", 1.0), ("charge", -1.0), ("mass", 1.0),
+                 ("dens", 1.0))
 
 
 class Particles(ParticleMethodsHIP):
@@ -49,28 +55,27 @@ class Particles(ParticleMethodsHIP):
 
     def _process_configs(self, configs_in):
         """particles.py:53-100."""
-        self.Args = LazyArgs(configs_in)
-        self.Args["Np"] = 0
-        self.Args["Np_stay"] = 0
-        for key, val in (("dt", 1.0), ("dx", 1.0), ("dr", 1.0), ("charge", -1.0), ("mass", 1.0),
-                         ("dens", 1.0)):
-            if key not in self.Args:
-                self.Args[key] = val
-        dt, q_s, m_s, n_s, dx, dr = [self.Args[k] for k in ("dt", "charge", "mass", "dens", "dx", "dr")]
-        self.Args["dt_2"] = 0.5 * dt
-        if "Nppc" in self.Args:
-            self.Args["Nppc"] = np.array(self.Args["Nppc"], dtype=np.uint32)
-            num_ppc = np.prod(self.Args["Nppc"])
-            self.Args["w0"] = 2 * np.pi * dx * dr * n_s / num_ppc
-            self.Args["ddx"] = dx / self.Args["Nppc"][0]
+        a = self.Args = LazyArgs(configs_in)
+        a["Np"] = a["Np_stay"] = 0
+        for key, default in _ARG_DEFAULTS:
+            a.setdefault(key, default)
+        a["dt_2"] = 0.5 * a["dt"]
+        # macro-particle weight of a cell-filled plasma: the charge of one
+        # cell of density dens, 2 pi dx dr (times r, applied per particle),
+        # shared among the prod(Nppc) particles of the cell
+        if "Nppc" in a:
+            a["Nppc"] = np.array(a["Nppc"], dtype=np.uint32)
+            a["w0"] = 2 * np.pi * a["dx"] * a["dr"] * a["dens"] / np.prod(a["Nppc"])
+            a["ddx"] = a["dx"] / a["Nppc"][0]
         else:
-            self.Args["ddx"] = 1.0
-        self.Args["FactorPush"] = 2 * np.pi * dt * q_s / m_s
-        self.Args["right_lim"] = 0.0
+            a["ddx"] = 1.0
+        # Boris factor 2 pi dt q/m (time in lambda0/c, fields in m c omega0/e)
+        a["FactorPush"] = 2 * np.pi * a["dt"] * a["charge"] / a["mass"]
+        a["right_lim"] = 0.0
+        a["w2pC"] = _W2PC
+        a["dont_send"] = ["InjectorSource", "charge", "mass", "dens", "Immobile", "w2pC"]
+        a["dont_keep"] = []
         self.flag_sorted = False
-        self.Args["w2pC"] = 4 * np.pi ** 2 * _M_E * _C ** 2 * _EPS0 * 1e6 / _E
-        self.Args["dont_send"] = ["InjectorSource", "charge", "mass", "dens", "Immobile", "w2pC"]
-        self.Args["dont_keep"] = []
 
     def _init_data_on_dev(self):
         """particles.py:102-114 (empty SoA arrays of the step's precision)."""

@@ -19,8 +19,17 @@ reference's loop_steps names.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import torch
+
+
+def counter_collection_active():
+    """True inside `rocprofv3 --pmc ...` (the profiler exports
+    ROCPROF_COUNTER_COLLECTION to the program it runs)."""
+    v = os.environ.get("ROCPROF_COUNTER_COLLECTION", "")
+    return v.strip().lower() not in ("", "0", "false", "no")
 
 loop_steps = ["frame", "push-x", "sort", "depose", "transform", "smooth", "data_copy",
               "grad", "push-eb", "damp-eb", "restore_B", "gather + push-p"]
@@ -195,6 +204,17 @@ class PIC_loop:
                             ("xh", "yh", "zh", "indx_in_cell") if consume else ())
 
     def _capture(self, key):
+        # A rocprofv3 counter-collection run of the graph-replayed step ended
+        # in a segmentation fault in one of the profiler's threads as soon as
+        # the first captured steps were replayed (the same steps run eagerly
+        # profile normally), so graph mode refuses to run under counter
+        # collection: collect counters with graph=False, which launches the
+        # same kernels in the same order.
+        if counter_collection_active():
+            raise RuntimeError(
+                "PIC_loop(graph=True) under rocprofv3 counter collection "
+                "(ROCPROF_COUNTER_COLLECTION is set): HIP graph replay is not supported by the "
+                "profiler's counter collection here; run with graph=False (eager steps)")
         # one eager step has run, so every buffer exists; ions stay sorted
         # (they are never pushed), electrons are sorted twice per step
         for parts in self.species:

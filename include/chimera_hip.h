@@ -477,6 +477,18 @@ int chm_pad_table(const double *src, int32_t M, int32_t K, double *d64,
  * the dDHT tables and by the tests. */
 int chm_gemm_f64(const double *A, const double *B, double *C, int32_t M,
                  int32_t K, int32_t N, void *stream);
+/* Measurement aid (no counterpart in the reference): chm_dht_timing(1)
+ * clears the record and makes every DHT product launch on a stream that is
+ * not being captured record a HIP event before and after it on that stream,
+ * with the launch's algorithmic flops (2 M K N per product; the FP32-
+ * equivalent work for split-BF16) and its form (0 FP64, 1 FP32, 2
+ * split-BF16); chm_dht_timing(0) stops and clears.
+ * chm_dht_timing_collect waits for the recorded launches, writes up to max
+ * (duration ms, flops, form) triples (any output may be NULL), clears the
+ * record and returns the count (-1 on a HIP error). */
+int chm_dht_timing(int32_t enable);
+int64_t chm_dht_timing_collect(double *ms, double *flops, int32_t *kind,
+                               int64_t max);
 /* Converts n doubles to float (round to nearest) and, if hi/lo are not NULL,
  * to a split-BF16 pair with hi + lo ~= src. */
 int chm_convert_table(const double *src, int64_t n, float *f32,
@@ -500,26 +512,32 @@ int chm_laser_spectral_f64(const chm_spectral_t *sp, const double *w_m0,
  * with fill_grid (kernels/particles_generic.cl:33-84): for every cell of the
  * local grids xgrid[Nx_loc] x rgrid (ncells = (Nx_loc-1)(Nr_loc-1)),
  * npt x npr x npx particles at the regular sub-positions, the theta set
- * rotated by a per-cell uniform offset in [0, 2 pi); w = r * w0; momenta
- * (px_c, py_c, pz_c) and g_inv = 1/sqrt(1 + p^2) when px is not NULL.
- * The offset of cell c is 2 pi u with u the top 53 bits of splitmix64 at
- * (seed, counter0 + c) (the reference draws it from PyOpenCL's generator,
- * which is not reproducible here); sin/cos are evaluated from IEEE +, -, *,
- * / only, so a CPU restatement reproduces every bit. */
+ * rotated by a per-cell uniform offset in [0, 2 pi); w = r * w0; when px
+ * is not NULL, momenta p_c + dp n per component (n a standard normal per
+ * particle and component; a zero spread dp gives p_c exactly) and
+ * g_inv = 1/sqrt(1 + p^2).  The offset of cell c is 2 pi u with u the top 53
+ * bits of splitmix64 at (seed, counter0 + c) (the reference draws it from
+ * PyOpenCL's generator, which is not reproducible here); with a nonzero
+ * spread, particle p draws its normals by Box-Muller from counters
+ * counter0 + ncells + 4p .. + 3, so a fill uses ncells + 4 Np counters (else
+ * ncells).  sin/cos/log are evaluated from IEEE +, -, *, / only, so a CPU
+ * restatement reproduces every bit. */
 int chm_fill_domain_f64(double *x, double *y, double *z, double *w, double *px,
                         double *py, double *pz, double *g_inv,
                         const double *xgrid, const double *rgrid,
                         int32_t Nx_loc, int64_t ncells, int32_t npx,
                         int32_t npr, int32_t npt, double w0, double px_c,
-                        double py_c, double pz_c, uint64_t seed,
-                        uint64_t counter0, void *stream);
+                        double py_c, double pz_c, double dpx, double dpy,
+                        double dpz, uint64_t seed, uint64_t counter0,
+                        void *stream);
 int chm_fill_domain_f32(float *x, float *y, float *z, float *w, float *px,
                         float *py, float *pz, float *g_inv,
                         const double *xgrid, const double *rgrid,
                         int32_t Nx_loc, int64_t ncells, int32_t npx,
                         int32_t npr, int32_t npt, double w0, double px_c,
-                        double py_c, double pz_c, uint64_t seed,
-                        uint64_t counter0, void *stream);
+                        double py_c, double pz_c, double dpx, double dpy,
+                        double dpz, uint64_t seed, uint64_t counter0,
+                        void *stream);
 /* ParticleMethodsCL.make_new_beam (methods/particles_methods_cl.py:149-177):
  * Np particles with positions c + L n, momenta pc + dp n (n standard
  * normals), weights w, g_inv = 1/sqrt(1 + p^2).  Particle p's six normals

@@ -353,14 +353,26 @@ class OracleParticles:
                "z": np.broadcast_to(rp * cos_th, shp).ravel().copy(),
                "w": np.broadcast_to(rp * a["w0"], shp).ravel().copy()}
         if not self.immobile:
+            # particles_methods_cl.py:114-147: p_c, or p_c + dp * randn when a
+            # spread dp is given; normals of particle p from counters
+            # counter + ncells + 4p .. + 2 (det_normal2 pairs), as the GPU draws them
             pc = [float(parts_in.get(k + "_c", 0.0)) for k in ("px", "py", "pz")]
+            dp = [float(parts_in.get("d" + k, 0.0)) for k in ("px", "py", "pz")]
             n = new["x"].size
-            for k, v in zip(("px", "py", "pz"), pc):
-                new[k] = np.full(n, v)
-            g = 1.0
-            if any(v != 0.0 for v in pc):
-                g = 1. / np.sqrt(1 + pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2])
-            new["g_inv"] = np.full(n, g)
+            if any(v != 0.0 for v in dp):
+                c = np.uint64(counter + ncells) + np.uint64(4) * np.arange(n, dtype=np.uint64)
+                n0, n1 = det_normal2(int(a.get("seed", 0)), c)
+                n2, _ = det_normal2(int(a.get("seed", 0)), c + np.uint64(2))
+                ux, uy, uz = pc[0] + dp[0] * n0, pc[1] + dp[1] * n1, pc[2] + dp[2] * n2
+                new.update(px=ux, py=uy, pz=uz, g_inv=1. / np.sqrt(1 + ux * ux + uy * uy + uz * uz))
+                a["rng_counter"] += 4 * n
+            else:
+                for k, v in zip(("px", "py", "pz"), pc):
+                    new[k] = np.full(n, v)
+                g = 1.0
+                if any(v != 0.0 for v in pc):
+                    g = 1. / np.sqrt(1 + pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2])
+                new["g_inv"] = np.full(n, g)
         for k, v in new.items():
             self.data[k + "_new"] = v
         if density_profiles is not None:

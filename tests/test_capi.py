@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     src = open(os.path.join(ROOT, "include", "chimera_hip.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(?:int|size_t)\s+(chm_\w+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(?:int|int64_t|size_t)\s+(chm_\w+)\s*\(", src)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -77,3 +77,10 @@ def test_noalloc_sort_rejects_oversize_without_gpu():
                                       None, None, None)
     assert rc == -4
 
+
+def test_dht_timing_record_empty_without_launches():
+    from chimeracl_amd import _lib
+    L = _lib.lib
+    assert L.chm_dht_timing(1) == 0
+    assert L.chm_dht_timing_collect(None, None, None, 10) == 0
+    assert L.chm_dht_timing(0) == 0

@@ -440,7 +440,7 @@ def test_push_x_and_profile_adjoints_fd(psim):
 
 
 # ---------------------------------------------------------------- whole run
-def _tiny_diff(precision="f64", nsteps=6, ckpt=4):
+def _tiny_diff(precision="f64", nsteps=6, ckpt=4, gather_skip="index"):
     import chimeracl_amd.adjoint as A
     from chimeracl_amd import config as C
     from chimeracl_amd.methods.generic_methods_hip import Communicator
@@ -456,15 +456,20 @@ def _tiny_diff(precision="f64", nsteps=6, ckpt=4):
     del s0
     plasma = C.load_plasma(g, (2, 2, 2), 0.01, seed=0)
     knots_x = np.linspace(g["Xmin"], g["Xmax"], 5)
-    sim = A.DiffPIC(g, las, (eo, io), plasma, knots_x, comm, ckpt=ckpt)
+    sim = A.DiffPIC(g, las, (eo, io), plasma, knots_x, comm, ckpt=ckpt, gather_skip=gather_skip)
     knots = 1.0 + 0.2 * np.random.default_rng(1).uniform(-1, 1, 5)
     return sim, las, knots
 
 
-def test_run_gradient_vs_finite_differences():
+@pytest.mark.parametrize("gather_skip", ["index", "cell"])
+def test_run_gradient_vs_finite_differences(gather_skip):
+    """The whole run's adjoint against central finite differences, with the
+    reference's gather skip (particles with index >= Np_stay are not pushed)
+    and with the differentiable form's alternative that pushes every particle
+    of a valid cell."""
     _require_gpu()
     import chimeracl_amd.adjoint as A
-    sim, las, knots = _tiny_diff()
+    sim, las, knots = _tiny_diff(gather_skip=gather_skip)
     nsteps = 6
     rg = sim.solver.Args["Rgrid"]
 
@@ -494,7 +499,7 @@ def test_run_gradient_vs_finite_differences():
     scale = np.abs(fd).max()
     err = np.abs(grads - fd) / scale
     print("adjoint", grads, "\nfd     ", fd, "\nerr", err)
-    assert float(L) > 0
+    assert float(L.detach()) > 0
     assert err.max() < 1e-5, err
 
 

@@ -40,6 +40,44 @@ def test_fill_domain_bit_exact(precision):
     assert g.Args["rng_counter"] == o.Args["rng_counter"]
 
 
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_fill_domain_thermal_bit_exact(precision):
+    """make_new_domain with thermal spreads (particles_methods_cl.py:114-147:
+    p = p_c + dp randn per component): the GPU fill equals the oracle's bit
+    for bit over two fills in a row (the counter advances by ncells + 4 Np),
+    and the momenta have the requested means and spreads."""
+    _require_gpu()
+    import chimeracl_amd as CA
+    from chimeracl_amd.methods.generic_methods_hip import Communicator
+    from oracle import oracle as O
+    cfg = {"Nppc": (2, 2, 4), "dx": 0.02, "dr": 0.1, "dt": 0.02, "dens": 0.002, "charge": -1,
+           "seed": 9}
+    comm = Communicator(precision=precision)
+    g = CA.Particles(dict(cfg), comm)
+    o = O.OracleParticles(dict(cfg))
+    dom = {"Xmin": -2.0, "Xmax": -0.5, "Rmin": 0.0, "Rmax": 4.0, "px_c": 0.2, "dpx": 0.05,
+           "dpy": 0.01, "dpz": 0.02}
+    for _ in range(2):
+        g.make_new_domain(dict(dom))
+        o.make_new_domain(dict(dom))
+        for k in ("x", "y", "z", "w", "px", "py", "pz", "g_inv"):
+            a = g.DataDev[k + "_new"].cpu().numpy()
+            b = o.data[k + "_new"]
+            if precision == "mixed":
+                b = b.astype(np.float32)
+            assert a.shape == b.shape and np.array_equal(a, b), k
+        assert g.Args["rng_counter"] == o.Args["rng_counter"]
+    n = g.DataDev["px_new"].numel()
+    for k, mu, sd in (("px", 0.2, 0.05), ("py", 0.0, 0.01), ("pz", 0.0, 0.02)):
+        v = g.DataDev[k + "_new"].double().cpu().numpy()
+        assert abs(v.mean() - mu) < 5 * sd / np.sqrt(n), k
+        assert abs(v.std() - sd) < 5 * sd / np.sqrt(2 * n), k
+    u = [g.DataDev[k + "_new"].double().cpu().numpy() for k in ("px", "py", "pz")]
+    gi = g.DataDev["g_inv_new"].double().cpu().numpy()
+    assert np.allclose(gi, 1 / np.sqrt(1 + u[0] ** 2 + u[1] ** 2 + u[2] ** 2),
+                       rtol=1e-6 if precision == "mixed" else 1e-15)
+
+
 def _frame_pair(precision="f64"):
     import chimeracl_amd as CA
     from chimeracl_amd import config as C

@@ -372,3 +372,34 @@ def test_beam_statistics(oracle_mod):
     assert np.allclose(d["g_inv_new"], 1 / np.sqrt(1 + u2))
     # two normals of a pair are uncorrelated
     assert abs(np.corrcoef(d["x_new"], d["y_new"])[0, 1]) < 5 / np.sqrt(Np)
+
+
+def test_make_new_domain_thermal_statistics(oracle_mod):
+    """make_new_domain with thermal spreads (particles_methods_cl.py:114-147):
+    each momentum component is p_c + dp N(0, 1); the sample means and
+    standard deviations match within 5 standard errors, the components are
+    uncorrelated, g_inv = 1/sqrt(1 + p^2), positions and weights are those of
+    the cold fill, and the counter advances by ncells + 4 Np."""
+    O = oracle_mod
+    cfg = {"Nppc": (2, 2, 4), "dx": 0.02, "dr": 0.1, "dt": 0.02, "dens": 0.002, "charge": -1,
+           "seed": 3}
+    dom = {"Xmin": -2.0, "Xmax": 0.0, "Rmin": 0.0, "Rmax": 4.0}
+    cold = O.OracleParticles(dict(cfg))
+    cold.make_new_domain(dict(dom))
+    p = O.OracleParticles(dict(cfg))
+    p.make_new_domain(dict(dom, px_c=0.5, dpx=0.1, dpz=0.03))
+    d = p.data
+    n = d["px_new"].size
+    ncells = n // 16
+    for k in ("x", "y", "z", "w"):
+        assert np.array_equal(d[k + "_new"], cold.data[k + "_new"]), k
+    se = 5 / np.sqrt(n)
+    for k, mu, sig in (("px", 0.5, 0.1), ("py", 0.0, 0.0), ("pz", 0.0, 0.03)):
+        v = d[k + "_new"]
+        assert abs(v.mean() - mu) <= se * sig + 1e-15, k
+        assert abs(v.std() - sig) <= se * sig / np.sqrt(2) * 2 + 1e-15, k
+    assert abs(np.corrcoef(d["px_new"], d["pz_new"])[0, 1]) < 5 / np.sqrt(n)
+    u2 = d["px_new"] ** 2 + d["py_new"] ** 2 + d["pz_new"] ** 2
+    assert np.allclose(d["g_inv_new"], 1 / np.sqrt(1 + u2), rtol=1e-15)
+    assert p.Args["rng_counter"] == ncells + 4 * n
+    assert cold.Args["rng_counter"] == ncells
