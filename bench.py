@@ -761,13 +761,13 @@ def diff_section():
     return out
 
 
-def aligned_section(args, k):
+def aligned_section(args, k, diag_at):
     """Both forms again with PIC_loop(align_every=k) (k = 0: never
     reordered): same workload, warmup and timed window as the main
-    measurement."""
+    measurement (timed from the first diagnostics step diag_at[0] on)."""
     import torch
     out = {"align_every": k}
-    first = min([int(v) for v in args.diag_steps.split(",") if v] or [0])
+    first = min(diag_at or [0])
     for name, prec in (("f64", "f64"), ("mixed", "mixed")):
         if prec == "mixed" and args.no_mixed:
             continue
@@ -1051,7 +1051,7 @@ def main():
     if args.secondary and args.secondary != args.config:
         second = measure(args, args.secondary, diag_for(args.secondary), False, None,
                          with_kernels=False, with_window=False)
-    aligned = (aligned_section(args, args.compare_align)
+    aligned = (aligned_section(args, args.compare_align, diag_for(args.config))
                if args.compare_align >= 0 and args.compare_align != args.align_every else None)
     configs = None
     if not args.no_configs and "C1" not in (args.config, args.secondary):

@@ -41,7 +41,8 @@ class Spectral(ctypes.Structure):
                 ("twiddle", P),
                 ("DHT", _MP), ("DHT_inv", _MP), ("dDp", _MP), ("dDm", _MP),
                 ("DHT_lo", _MP), ("DHT_inv_lo", _MP), ("dDp_lo", _MP), ("dDm_lo", _MP),
-                ("c1", _MP), ("c2", _MP), ("c3", _MP), ("smooth", _MP)]
+                ("c1", _MP), ("c2", _MP), ("c3", _MP), ("smooth", _MP),
+                ("c1d", _MP), ("c2d", _MP), ("c3d", _MP)]
 
 
 class TablesOut(ctypes.Structure):

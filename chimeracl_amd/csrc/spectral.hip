@@ -72,6 +72,48 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_rows(FftBatch<T> fb) {
   }
 }
 
+// Rows of 4096: the register-resident FFT (fft4096_reg), one row per block
+// of 256 threads, thread t handling elements t + 256 j; same loads, phases,
+// scale, table and stores as k_fft_rows.
+template <typename T, int SIGN>
+__global__ void __launch_bounds__(256) k_fft_rows4096(FftBatch<T> fb) {
+  typedef typename C2<T>::t T2;
+  constexpr int N = 4096;
+  __shared__ T sbuf[FFT4096_LDS];
+  const FftPlane<T> &pl = fb.p[blockIdx.y];
+  const int row = blockIdx.x;
+  const int64_t off = (int64_t)row * N;
+  const T2 *phin = (const T2 *)fb.phase_in, *phout = (const T2 *)fb.phase_out;
+  T2 v[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int x = threadIdx.x + 256 * j;
+    T2 a;
+    a.x = 0;
+    a.y = 0;
+    if (pl.in_cplx) a = ((const T2 *)pl.in)[off + x];
+    else a.x = pl.in[off + x];
+    if (phin) a = c_mul(a, phin[x]);
+    v[j] = a;
+  }
+  fft4096_reg<SIGN, T>(v, sbuf, (const T2 *)fb.tw);
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int x = threadIdx.x + 256 * j;
+    T2 a = v[j];
+    a.x *= fb.scale;
+    a.y *= fb.scale;
+    if (phout) a = c_mul(a, phout[x]);
+    if (pl.table) {
+      T tb = pl.table[off + x];
+      a.x = tb * a.x;
+      a.y = tb * a.y;
+    }
+    if (pl.out_cplx) ((T2 *)pl.out)[off + x] = a;
+    else pl.out[off + x] = a.x;
+  }
+}
+
 template <typename T, int SIGN>
 static int launch_fft(const FftBatch<T> &fb, int nplanes, int N, hipStream_t st) {
   if (nplanes <= 0) return CHM_OK;
@@ -82,8 +124,11 @@ static int launch_fft(const FftBatch<T> &fb, int nplanes, int N, hipStream_t st)
     k_fft_rows<T, NN, SIGN><<<g, FFT_NT, 0, st>>>(fb);                                        \
     break;                                                                                    \
   }
-    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048) CASE(4096)
+    CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048)
 #undef CASE
+  case 4096:
+    k_fft_rows4096<T, SIGN><<<dim3(fb.nrows, nplanes), 256, 0, st>>>(fb);
+    break;
   default:
     return CHM_EUNSUPPORTED;
   }
@@ -456,6 +501,7 @@ struct SolvPlanes {
   void *e[3][CHM_MAXM + 1], *g[3][CHM_MAXM + 1];
   const void *j[3][CHM_MAXM + 1], *n0[3][CHM_MAXM + 1], *n1[3][CHM_MAXM + 1];
   const void *c1[CHM_MAXM + 1], *c2[CHM_MAXM + 1], *c3[CHM_MAXM + 1];
+  int c64;  // the c tables are double (always for T = double; optional for float)
 };
 
 // advance_e_g_m, solver_ms_pic.cl:187-213 (same expression order)
@@ -470,18 +516,39 @@ __device__ __forceinline__ void psatd(T c1, T c2, T c3, T dt_inv, T e0, T g0, T 
   g1 = -c2 * e0 + c1 * (g0 - j0) + j0 + c3 * (dt_inv * ((T)1. - c1) * (n0 - n1) - c2 * n0);
 }
 
+// PSATD of one complex element: with double coefficient tables (FP64, or
+// the FP32 form given FP64 tables) the update runs in double and the result
+// is rounded to T once.
+template <typename T>
+__device__ __forceinline__ void psatd_elem(const SolvPlanes &sv, int m, int c, int64_t id, T dt_inv,
+                                           typename C2<T>::t e0, typename C2<T>::t g0,
+                                           typename C2<T>::t &e1, typename C2<T>::t &g1) {
+  typedef typename C2<T>::t T2;
+  T2 j0 = ((const T2 *)sv.j[c][m])[id], n0 = ((const T2 *)sv.n0[c][m])[id], n1 = ((const T2 *)sv.n1[c][m])[id];
+  if (sizeof(T) == 4 && sv.c64) {
+    const double c1 = ((const double *)sv.c1[m])[id], c2 = ((const double *)sv.c2[m])[id],
+                 c3 = ((const double *)sv.c3[m])[id];
+    double ex, gx, ey, gy;
+    psatd<double>(c1, c2, c3, (double)dt_inv, e0.x, g0.x, j0.x, n0.x, n1.x, ex, gx);
+    psatd<double>(c1, c2, c3, (double)dt_inv, e0.y, g0.y, j0.y, n0.y, n1.y, ey, gy);
+    e1.x = (T)ex; e1.y = (T)ey;
+    g1.x = (T)gx; g1.y = (T)gy;
+  } else {
+    T c1 = ((const T *)sv.c1[m])[id], c2 = ((const T *)sv.c2[m])[id], c3 = ((const T *)sv.c3[m])[id];
+    psatd<T>(c1, c2, c3, dt_inv, e0.x, g0.x, j0.x, n0.x, n1.x, e1.x, g1.x);
+    psatd<T>(c1, c2, c3, dt_inv, e0.y, g0.y, j0.y, n0.y, n1.y, e1.y, g1.y);
+  }
+}
+
 template <typename T> __global__ void k_advance(SolvPlanes sv, T dt_inv, int64_t n) {
   typedef typename C2<T>::t T2;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int m = blockIdx.y;
   if (e >= n) return;
-  T c1 = ((const T *)sv.c1[m])[e], c2 = ((const T *)sv.c2[m])[e], c3 = ((const T *)sv.c3[m])[e];
   for (int k = 0; k < 3; k++) {
     T2 e0 = ((T2 *)sv.e[k][m])[e], g0 = ((T2 *)sv.g[k][m])[e];
-    T2 j0 = ((const T2 *)sv.j[k][m])[e], n0 = ((const T2 *)sv.n0[k][m])[e], n1 = ((const T2 *)sv.n1[k][m])[e];
     T2 e1, g1;
-    psatd<T>(c1, c2, c3, dt_inv, e0.x, g0.x, j0.x, n0.x, n1.x, e1.x, g1.x);
-    psatd<T>(c1, c2, c3, dt_inv, e0.y, g0.y, j0.y, n0.y, n1.y, e1.y, g1.y);
+    psatd_elem<T>(sv, m, k, e, dt_inv, e0, g0, e1, g1);
     ((T2 *)sv.e[k][m])[e] = e1;
     ((T2 *)sv.g[k][m])[e] = g1;
   }
@@ -502,6 +569,18 @@ static void fill_solv(SolvPlanes &sv, const chm_spectral_t *sp, const chm_fb_t *
     sv.c1[m] = sp->c1[m];
     sv.c2[m] = sp->c2[m];
     sv.c3[m] = sp->c3[m];
+  }
+  // the FP32 forms take the FP64 coefficient tables when all are given
+  bool d = true;
+  for (int m = 0; m <= sp->M; m++) d = d && sp->c1d[m] && sp->c2d[m] && sp->c3d[m];
+  sv.c64 = 0;
+  if (d) {
+    for (int m = 0; m <= sp->M; m++) {
+      sv.c1[m] = sp->c1d[m];
+      sv.c2[m] = sp->c2d[m];
+      sv.c3[m] = sp->c3d[m];
+    }
+    sv.c64 = 1;
   }
 }
 
@@ -531,21 +610,25 @@ extern "C" int chm_advance_fields_f32(const chm_spectral_t *sp, const chm_fb_t *
 
 // ---------------------------------------------------------------- damping
 // One block per (row kr, mode m, component c): optional PSATD of the row,
-// then for E and G: *phase_plus -> IFFT -> (m=0: real part) -> *profile(x)
-// -> FFT -> *phase_minus.  (Solver.damp_fields, solver.py:32-35.)
+// then for E and G the damping round trip (Solver.damp_fields,
+// solver.py:32-35): *phase_plus -> IFFT -> (m=0: real part) -> *profile(x)
+// -> FFT -> *phase_minus.  The forward kernel computes it in correction
+// form, row - FFT(v - f g(v)) phase_minus with v = IFFT(row phase_plus)/N,
+// g = Re for m = 0 (identity otherwise) and f the profile: the same map in
+// exact arithmetic, but the rounding of the transforms and of the phase
+// factors then only touches the correction, which vanishes where f = 1
+// (away from the edges), instead of the whole row in every step.
 // Rows of 4096 run with 512 threads (one radix-8 butterfly per thread and
-// pass, eight row elements per thread), shorter rows with 256.  The E row
-// goes straight into the LDS buffer and only the G row waits in registers,
-// so that a thread holds one row share at a time (at 4096 x FP64 the
-// earlier form that held both in registers spilled to scratch).
+// pass, eight row elements per thread), shorter rows with 256.
 template <int N> struct DampCfg {
   static constexpr int NT = N >= 4096 ? 512 : FFT_NT;
   static constexpr int PER = (N + NT - 1) / NT;
 };
 
 // The damping round trip of the row in buf (already multiplied by
-// phase_plus), in place; the map is self-adjoint, so the adjoint kernel
-// applies it unchanged.  Call after a barrier; returns after one.
+// phase_plus), in place (direct form; used by the adjoint: the map is
+// self-adjoint, so the adjoint kernel applies it unchanged).  Call after a
+// barrier; returns after one.
 template <typename T, int N, int NT>
 __device__ __forceinline__ void damp_lds(typename C2<T>::t *buf, int m, const T *__restrict__ damp,
                                          int nd, const T *__restrict__ tw) {
@@ -565,11 +648,30 @@ __device__ __forceinline__ void damp_lds(typename C2<T>::t *buf, int m, const T 
   fft_lds<T, N, -1, 1, NT>(buf, (const T2 *)tw);
 }
 
-// GREG: the PSATD's new G row waits in registers while E is damped (no
-// extra traffic); otherwise it is stored in place right away and read back
-// for its own round trip (no register array; the reread is of data this
-// block has just written).
-template <int N, bool ADV> struct DampG { static constexpr bool GREG = !ADV; };
+// The correction of the round trip: buf (row * phase_plus) becomes
+// FFT(v - f g(v)), v = IFFT(buf)/N.  Call after a barrier; returns after one.
+template <typename T, int N, int NT>
+__device__ __forceinline__ void damp_corr_lds(typename C2<T>::t *buf, int m,
+                                              const T *__restrict__ damp, int nd,
+                                              const T *__restrict__ tw) {
+  typedef typename C2<T>::t T2;
+  const T inv = (T)1 / (T)N;
+  fft_lds<T, N, 1, 1, NT>(buf, (const T2 *)tw);
+  for (int x = threadIdx.x; x < N; x += NT) {
+    T2 v = buf[x];
+    v.x *= inv;
+    v.y *= inv;
+    T f = (T)1;
+    if (x < nd) f *= damp[x];
+    if (x > N - nd) f *= damp[N - x];
+    T2 r;
+    r.x = v.x - f * v.x;
+    r.y = m == 0 ? v.y : v.y - f * v.y;
+    buf[x] = r;
+  }
+  __syncthreads();
+  fft_lds<T, N, -1, 1, NT>(buf, (const T2 *)tw);
+}
 
 template <typename T, int N, bool ADV>
 __global__ void __launch_bounds__(DampCfg<N>::NT) k_damp(SolvPlanes sv, const T *__restrict__ damp,
@@ -578,45 +680,48 @@ __global__ void __launch_bounds__(DampCfg<N>::NT) k_damp(SolvPlanes sv, const T 
                                                          const T *__restrict__ tw, T dt_inv, int Nk) {
   typedef typename C2<T>::t T2;
   constexpr int NT = DampCfg<N>::NT, PER = DampCfg<N>::PER;
-  constexpr bool GREG = DampG<N, ADV>::GREG;
   __shared__ T2 buf[N];
   const int kr = blockIdx.x, m = blockIdx.y, c = blockIdx.z;
   const int64_t rowoff = (int64_t)kr * N;
   const T2 *pp = (const T2 *)php, *pm = (const T2 *)phm;
   T2 *E = (T2 *)sv.e[c][m] + rowoff, *G = (T2 *)sv.g[c][m] + rowoff;
-  T2 gv[GREG ? PER : 1];
+  // the new E and G go to memory right away; each thread later rereads
+  // only the elements it wrote itself (x = threadIdx.x + q NT throughout)
 #pragma unroll
   for (int q = 0; q < PER; q++) {
     const int x = threadIdx.x + q * NT;
     if (x < N) {
-      T2 e0 = E[x], g0 = G[x];
+      T2 e0 = E[x];
       if (ADV) {
-        const int64_t id = rowoff + x;
-        T c1 = ((const T *)sv.c1[m])[id], c2 = ((const T *)sv.c2[m])[id], c3 = ((const T *)sv.c3[m])[id];
-        T2 j0 = ((const T2 *)sv.j[c][m])[id], n0 = ((const T2 *)sv.n0[c][m])[id], n1 = ((const T2 *)sv.n1[c][m])[id];
         T2 e1, g1;
-        psatd<T>(c1, c2, c3, dt_inv, e0.x, g0.x, j0.x, n0.x, n1.x, e1.x, g1.x);
-        psatd<T>(c1, c2, c3, dt_inv, e0.y, g0.y, j0.y, n0.y, n1.y, e1.y, g1.y);
+        psatd_elem<T>(sv, m, c, rowoff + x, dt_inv, e0, G[x], e1, g1);
+        E[x] = e1;
+        G[x] = g1;
         e0 = e1;
-        g0 = g1;
       }
       buf[x] = c_mul(e0, pp[x]);
-      if constexpr (GREG) gv[q] = g0;
-      else G[x] = g0;
     }
   }
   __syncthreads();
-  damp_lds<T, N, NT>(buf, m, damp, nd, tw);
-  for (int x = threadIdx.x; x < N; x += NT) E[x] = c_mul(buf[x], pm[x]);
+  damp_corr_lds<T, N, NT>(buf, m, damp, nd, tw);
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int x = threadIdx.x + q * NT;
+    if (x < N) E[x] = c_sub(E[x], c_mul(buf[x], pm[x]));
+  }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < PER; q++) {
     const int x = threadIdx.x + q * NT;
-    if (x < N) buf[x] = c_mul(GREG ? gv[q] : G[x], pp[x]);  // G[x]: this thread's own store
+    if (x < N) buf[x] = c_mul(G[x], pp[x]);
   }
   __syncthreads();
-  damp_lds<T, N, NT>(buf, m, damp, nd, tw);
-  for (int x = threadIdx.x; x < N; x += NT) G[x] = c_mul(buf[x], pm[x]);
+  damp_corr_lds<T, N, NT>(buf, m, damp, nd, tw);
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int x = threadIdx.x + q * NT;
+    if (x < N) G[x] = c_sub(G[x], c_mul(buf[x], pm[x]));
+  }
 }
 
 template <typename T>

@@ -122,6 +122,13 @@ class TransformerMethodsHIP(GenericMethodsHIP):
                 sp.dDp_lo[m], sp.dDm_lo[m] = ptr(src[f"dDHT_plus_m{m}_lo"]), ptr(src[f"dDHT_minus_m{m}_lo"])
             sp.c1[m], sp.c2[m] = ptr(src[f"MxSlv_cos(wdt)_m{m}"]), ptr(src[f"MxSlv_sin(wdt)*w_m{m}"])
             sp.c3[m], sp.smooth[m] = ptr(src[f"MxSlv_1/w**2_m{m}"]), ptr(src[f"SmoothingFilter_m{m}"])
+            if self.comm.mixed:
+                # the FP64 PSATD coefficients (the update of the FP32 fields
+                # runs in FP64 arithmetic from them: no systematic rotation
+                # error from rounding the tables)
+                sp.c1d[m] = ptr(D[f"MxSlv_cos(wdt)_m{m}"])
+                sp.c2d[m] = ptr(D[f"MxSlv_sin(wdt)*w_m{m}"])
+                sp.c3d[m] = ptr(D[f"MxSlv_1/w**2_m{m}"])
         self._sp = sp
         Nx, Nr, es = a["Nx"], a["Nr"], 4 if self.comm.mixed else 8
         nws = max(_lib.lib.chm_transform_workspace_bytes(Nx, Nr, M, 10, es),

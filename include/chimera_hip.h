@@ -130,6 +130,14 @@ typedef struct chm_spectral {
   const void *c2[CHM_MAXM + 1];     /* w sin(w dt)  [Nk][Nx]              */
   const void *c3[CHM_MAXM + 1];     /* 1/w^2        [Nk][Nx] (= Poisson)  */
   const void *smooth[CHM_MAXM + 1]; /* smoothing filter [Nk][Nx]          */
+  /* _f32 only, optional: the PSATD coefficients c1, c2, c3 in FP64; when
+   * set, the FP32 forms' PSATD update (chm_advance_fields_f32,
+   * chm_damp_fields_f32 with advance_first) runs in FP64 arithmetic from
+   * them, so that the rounding of the coefficient tables does not act as
+   * the same systematic rotation error in every step. */
+  const double *c1d[CHM_MAXM + 1];
+  const double *c2d[CHM_MAXM + 1];
+  const double *c3d[CHM_MAXM + 1];
 } chm_spectral_t;
 
 /* Outputs of chm_build_tables_f64 (device pointers, all double). */

@@ -27,8 +27,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     from chimeracl_amd import _lib
-    # chm_spectral_t: 6 int32 + double + 5 pointers + 12 arrays of 5 pointers
-    assert ctypes.sizeof(_lib.Spectral) == 6 * 4 + 8 + 5 * 8 + 12 * 5 * 8
+    # chm_spectral_t: 6 int32 + double + 5 pointers + 15 arrays of 5 pointers
+    assert ctypes.sizeof(_lib.Spectral) == 6 * 4 + 8 + 5 * 8 + 15 * 5 * 8
     assert ctypes.sizeof(_lib.Grid) == 16 + 32
     assert ctypes.sizeof(_lib.Planes) == 8 * 5
     assert ctypes.sizeof(_lib.TablesOut) == 6 * 8 + 9 * 5 * 8

@@ -361,12 +361,17 @@ def test_tables(env, gi):
     assert np.allclose(D["phase_minus"].cpu().numpy(), s.phase(0), rtol=0, atol=1e-14)
 
 
-@pytest.mark.parametrize("gi", range(len(GRIDS)))
+# the spectral operators also at 4096-point rows (the register-resident
+# row FFT) with three modes
+SPEC_GRIDS = GRIDS + [dict(Nx=4096, Nr=10, M=2)]
+
+
+@pytest.mark.parametrize("gi", range(len(SPEC_GRIDS)))
 def test_spectral_ops(env, gi):
     import chimeracl_amd as CA
     from chimeracl_amd.methods.generic_methods_hip import Communicator
     from oracle import oracle as O
-    g = env.tiny_grid(**GRIDS[gi])
+    g = env.tiny_grid(**SPEC_GRIDS[gi])
     M = g["M"]
     solver = CA.Solver(dict(g), Communicator(precision="f64"))
     s = O.OracleSolver(dict(g))
@@ -466,3 +471,34 @@ def test_dht_product_vs_numpy(env):
         err = np.abs(out[rows] - ref[:rows]).max() / np.abs(ref[:rows]).max()
         assert err < 1e-13, (rows, err)
     assert np.array_equal(out[M][:64], out[64])
+
+
+@pytest.mark.parametrize("Nx", [2048, 4096])
+def test_mixed_transforms_vs_oracle(env, Nx):
+    """The FP32 row FFTs (LDS Stockham at 2048, register-resident at 4096)
+    with the split-BF16 DHT: forward and backward transforms of random
+    fields against the FP64 oracle within 2e-5 relative L2."""
+    import chimeracl_amd as CA
+    from chimeracl_amd.methods.generic_methods_hip import Communicator
+    from oracle import oracle as O
+    g = env.tiny_grid(Nx=Nx, Nr=12, M=1)
+    solver = CA.Solver(dict(g), Communicator(precision="mixed"))
+    s = O.OracleSolver(dict(g))
+    rng = np.random.default_rng(7)
+    for m in range(2):
+        shp = s.f[f"rho_m{m}"].shape
+        v = rng.normal(0, 1, shp) + (1j * rng.normal(0, 1, shp) if m else 0)
+        s.f[f"rho_m{m}"][...] = v
+        solver.DataDev[f"rho_m{m}"].copy_(torch.from_numpy(np.ascontiguousarray(v)))
+    solver.fb_transform(scals=["rho"], dir=0)
+    s.fb_transform(scals=["rho"], dir=0)
+    for m in range(2):
+        nm = f"rho_fb_m{m}"
+        assert env.rel_l2(env.field_host(solver, nm).astype(np.complex128), s.f[nm]) < 2e-5, nm
+    for m in range(2):
+        solver.DataDev[f"rho_m{m}"].zero_()
+    solver.fb_transform(scals=["rho"], dir=1)
+    s.fb_transform(scals=["rho"], dir=1)
+    for m in range(2):
+        nm = f"rho_m{m}"
+        assert env.rel_l2(env.field_host(solver, nm)[1:], s.f[nm][1:]) < 2e-5, nm
