@@ -787,11 +787,16 @@ __global__ void __launch_bounds__(256)
 // bit for bit.
 #define GS_NT 256
 template <typename T, int M> struct GatherCfg {
-  // two sorted slots per thread (their loads in flight together), registers
-  // held to 5 (FP64, M = 1; 4 at M = 0) / 6 (FP32) waves per SIMD, no spills; no
-  // bound above M = 1 (measured on the C2 step)
+  // two sorted slots per thread (their loads in flight together); registers
+  // held to a number of waves per SIMD measured on the 2048 x 256 step (M = 1:
+  // 5 FP64, 4 at M = 0, 6 FP32, no spills) and the 4096 x 512 step (M = 2:
+  // 4 FP64, a few spilled words, 4% faster than the unbounded 3 waves; 6 FP32
+  // for the pre-pushing gather, 13% faster than 4-5); no bound above M = 2
   static constexpr int IT = 2;
-  static constexpr int WPE = M <= 1 ? (sizeof(T) == 8 ? (M == 1 ? 5 : 4) : 6) : 1;
+  static constexpr int wpe(bool pre) {
+    return M <= 1 ? (sizeof(T) == 8 ? (M == 1 ? 5 : 4) : 6)
+                  : (M == 2 ? (sizeof(T) == 8 ? 4 : (pre ? 6 : 1)) : 1);
+  }
   static constexpr int NQ = 1 + 2 * M;
   static constexpr int NV = 6 * NQ;  // E and B, 3 components, all modes
   static constexpr int MN = 20480 / (2 * NV * (int)sizeof(T));
@@ -888,7 +893,7 @@ __device__ __forceinline__ void pre_push(const PrePush<T> &pp, uint32_t p, T xp,
 }
 
 template <typename T, int M, bool PRE>
-__global__ void __launch_bounds__(GS_NT) __attribute__((amdgpu_waves_per_eu(GatherCfg<T, M>::WPE)))
+__global__ void __launch_bounds__(GS_NT) __attribute__((amdgpu_waves_per_eu(GatherCfg<T, M>::wpe(PRE))))
     k_gather_sorted(const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
                     T *__restrict__ px, T *__restrict__ py, T *__restrict__ pz,
                     T *__restrict__ g_inv, const uint32_t *__restrict__ sort_indx,

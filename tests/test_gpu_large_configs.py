@@ -10,8 +10,10 @@
   the FP64 adjoint of a 10-step run (checkpoints every 5 steps) against
   central finite differences of the forward run for a0, w0 and three knots;
 * large differentiable step (4096 x 256, m = 0, 1, 2x2x8, 64 knots): the
-  mixed-precision adjoint against the FP64 adjoint after 20 steps, and the
-  backward / forward time ratio.
+  mixed-precision adjoint against the FP64 adjoint after 20 steps, the
+  backward / forward time ratio, and the FP64 adjoint of a 300-step run
+  against central differences at steps small enough for the chaotic
+  objective.
 """
 import numpy as np
 import pytest
@@ -213,3 +215,35 @@ def test_c5_mixed_adjoint_vs_fp64_adjoint():
     print(errs, ratio)
     assert max(errs.values()) < 1e-3, errs
     assert max(ratio.values()) <= 4.0, ratio
+
+
+def test_c5_fp64_adjoint_vs_fine_differences_300_steps():
+    """The FP64 adjoint over a long horizon (300 steps of the large
+    differentiable workload, checkpoints every 20 steps) against central
+    differences of the forward run.  The wake energy of the last 256 columns
+    is a chaotic function of a0 at this horizon: its derivative changes sign
+    and size over relative changes of a0 above ~1e-9 (central differences at
+    1e-2 .. 1e-5 relative steps give +0.49 .. +1.7, the derivative is about
+    -14.5), so the differences are taken at 1e-10 and 3e-10 relative steps,
+    where they have settled (FP64 rounding of the objective, ~1e-15, is far
+    below h * 14.5).  Bar: both within 1% of the adjoint."""
+    _require_gpu()
+    import chimeracl_amd.adjoint as A
+    print("C5 FP64 adjoint vs central differences at 1e-10, 300 steps", flush=True)
+    sim, a0, R, kv, _ = A.from_config("C5", "f64")
+    rg = sim.solver.Args["Rgrid"]
+    obj = lambda outs: A.objective_wake_energy(list(outs[3:]), rg, ncols=256)
+    n = 300
+    sim.forward(a0, R, kv, 2)
+    L, g = _grads(sim, a0, R, kv, n, obj)
+    fd = {}
+    for rel in (1e-10, 3e-10):
+        h = rel * a0
+        with torch.no_grad():
+            fp = float(obj(sim.forward(a0 + h, R, kv, n)))
+            fm = float(obj(sim.forward(a0 - h, R, kv, n)))
+        fd[rel] = (fp - fm) / (2 * h)
+    print(L, g[0], fd)
+    assert abs(g[0]) > 1.0   # the long-horizon derivative, not a small one
+    bad = {k: v for k, v in fd.items() if not abs(v - g[0]) < 1e-2 * abs(g[0])}
+    assert not bad, (g[0], bad)
