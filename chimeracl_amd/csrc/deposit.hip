@@ -287,6 +287,12 @@ __global__ void __launch_bounds__(256)
 }
 
 // ------------------------------------------------------------------ pass 2
+// The node's (up to) four cells are found first (their offsets loaded
+// together); then, component by component, the partial sums of all four are
+// loaded together before they are added in colour order: the two cells of a
+// column pair are read from the same lines of a corner plane (even cells by
+// one, odd by the other), so the lines are still in the L2 for the second
+// read instead of being fetched from HBM twice.
 template <typename T, int M, int NK>
 __global__ void __launch_bounds__(256)
     k_node_add(const uint32_t *__restrict__ cell_offset, const T *__restrict__ part, int Nx, int Nr,
@@ -297,6 +303,34 @@ __global__ void __launch_bounds__(256)
   if (node >= (int64_t)(Nr - 1) * Nx) return;
   int ir = (int)(node / Nx), ix = (int)(node - (int64_t)ir * Nx);
   constexpr int NQ = 1 + 2 * M;
+  int64_t cc[4];
+  int corner[4];
+  bool use[4];
+#pragma unroll
+  for (int col = 0; col < 4; col++) {
+    const int px = col & 1, pr = col >> 1;
+    const int cx = ((ix & 1) == px) ? ix : ix - 1;
+    const int cr = ((ir & 1) == pr) ? ir : ir - 1;
+    use[col] = cx > 0 && cx < Nx_cell - 1 && cr >= 0 && cr < Nr_cell - 1;
+    cc[col] = use[col] ? cx + (int64_t)cr * Nx_cell : 0;
+    corner[col] = 2 * (ir - cr) + (ix - cx);
+  }
+  uint32_t lo[4], hi[4];
+#pragma unroll
+  for (int col = 0; col < 4; col++) {
+    lo[col] = hi[col] = 0u;
+    if (use[col]) {
+      lo[col] = cell_offset[cc[col]];
+      hi[col] = cell_offset[cc[col] + 1];
+    }
+  }
+  bool any = false;
+#pragma unroll
+  for (int col = 0; col < 4; col++) {
+    use[col] = use[col] && lo[col] != hi[col];
+    any = any || use[col];
+  }
+  if (!any) return;
   T acc[NK][NQ];
 #pragma unroll
   for (int k = 0; k < NK; k++) {
@@ -308,23 +342,20 @@ __global__ void __launch_bounds__(256)
       acc[k][2 * m] = pc[2 * node + 1];
     }
   }
-  bool any = false;
 #pragma unroll
-  for (int col = 0; col < 4; col++) {
-    int px = col & 1, pr = col >> 1;
-    int cx = ((ix & 1) == px) ? ix : ix - 1;
-    int cr = ((ir & 1) == pr) ? ir : ir - 1;
-    if (!(cx > 0 && cx < Nx_cell - 1 && cr >= 0 && cr < Nr_cell - 1)) continue;
-    int64_t c = cx + (int64_t)cr * Nx_cell;
-    if (cell_offset[c] == cell_offset[c + 1]) continue;
-    int corner = 2 * (ir - cr) + (ix - cx);
-    any = true;
+  for (int k = 0; k < NK; k++) {
+    T v[4][NQ];
 #pragma unroll
-    for (int k = 0; k < NK; k++)
+    for (int col = 0; col < 4; col++)
 #pragma unroll
-      for (int q = 0; q < NQ; q++) acc[k][q] = acc[k][q] + part[((int64_t)(k * NQ + q) * 4 + corner) * ncell + c];
+      for (int q = 0; q < NQ; q++)
+        v[col][q] = use[col] ? part[((int64_t)(k * NQ + q) * 4 + corner[col]) * ncell + cc[col]] : (T)0;
+#pragma unroll
+    for (int col = 0; col < 4; col++)
+      if (use[col])
+#pragma unroll
+        for (int q = 0; q < NQ; q++) acc[k][q] = acc[k][q] + v[col][q];
   }
-  if (!any) return;
 #pragma unroll
   for (int k = 0; k < NK; k++) {
     ((T *)planes.m0[k])[node] = acc[k][0];

@@ -38,9 +38,22 @@ typedef unsigned short us4 __attribute__((ext_vector_type(4)));
 // Elements B[k][n], B[k][n + 1] of an even column n (zero outside K x N):
 // one 2-element load of the real field, or one complex element (re, im) of
 // each operand.  bmode 0 takes the vector load when ldb is even (aligned).
+// A thread's column is the same in every K-step: its complex-column indices
+// in X and Y (through the m = -1 map where asked) come from bcols, once.
 template <typename T> struct pair_t { T a, b; };
+struct BColIdx { int x, y; };
+__device__ __forceinline__ BColIdx bcols(const GemmBatch &gb, const GemmItem &it, int n) {
+  const int Nx = gb.Nx, c = n >> 1;
+  BColIdx r = {c, c};
+  if (gb.bmode == 1 && Nx > 0) {
+    if (it.remapX) r.x = (Nx - c) % Nx;
+    if (it.remapY) r.y = (Nx - c) % Nx;
+  }
+  return r;
+}
 template <typename T>
-__device__ __forceinline__ pair_t<T> load_b2(const GemmBatch &gb, const GemmItem &it, int k, int n) {
+__device__ __forceinline__ pair_t<T> load_b2(const GemmBatch &gb, const GemmItem &it, int k, int n,
+                                            BColIdx cx) {
   pair_t<T> r = {(T)0, (T)0};
   if (k >= gb.K || n >= gb.N) return r;
   if (gb.bmode == 0) {
@@ -57,17 +70,15 @@ __device__ __forceinline__ pair_t<T> load_b2(const GemmBatch &gb, const GemmItem
     return r;
   }
   typedef T t2 __attribute__((ext_vector_type(2)));
-  const int Nx = gb.Nx, x = n >> 1;
+  const int Nx = gb.Nx;
   {
-    const int xs = it.remapX ? ((Nx - x) % Nx) : x;
-    const t2 v = ((const t2 *)it.X)[(int64_t)k * Nx + xs];
+    const t2 v = ((const t2 *)it.X)[(int64_t)k * Nx + cx.x];
     const T a = it.remapX ? -v.x : v.x, b = v.y;  // -conj(F(-kx)): (-re, +im)
     r.a = (T)it.alpha.x * a - (T)it.alpha.y * b;
     r.b = (T)it.alpha.x * b + (T)it.alpha.y * a;
   }
   if (it.Y) {
-    const int xs = it.remapY ? ((Nx - x) % Nx) : x;
-    const t2 v = ((const t2 *)it.Y)[(int64_t)k * Nx + xs];
+    const t2 v = ((const t2 *)it.Y)[(int64_t)k * Nx + cx.y];
     const T a = it.remapY ? -v.x : v.x, b = v.y;
     r.a += (T)it.beta.x * a - (T)it.beta.y * b;
     r.b += (T)it.beta.x * b + (T)it.beta.y * a;
@@ -94,12 +105,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
   const int bp = tid & 31, bk = tid >> 5;
   T ra[4];
   pair_t<T> rb[2];
+  const BColIdx bcx = bcols(gb, it, n0 + 2 * bp);
   auto gload = [&](int k0) {
     const T *pa = A + (int64_t)(m0 + ar) * gb.lda + k0 + ak;
 #pragma unroll
     for (int q = 0; q < 4; q++) ra[q] = pa[q];
 #pragma unroll
-    for (int q = 0; q < 2; q++) rb[q] = load_b2<T>(gb, it, k0 + bk + 8 * q, n0 + 2 * bp);
+    for (int q = 0; q < 2; q++) rb[q] = load_b2<T>(gb, it, k0 + bk + 8 * q, n0 + 2 * bp, bcx);
   };
   typedef typename std::conditional<sizeof(T) == 8, d4, f4>::type acc_t;
   acc_t acc[2][2];
@@ -179,6 +191,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
   const int bp = tid & 31, bk = (tid >> 5) * NB;
   us8 rah[NA], ral[NA];
   pair_t<float> rb[NB];
+  const BColIdx bcx = bcols(gb, it, n0 + 2 * bp);
   auto gload = [&](int k0) {
     const int64_t o = (int64_t)(m0 + ar) * gb.lda + k0 + ak;
 #pragma unroll
@@ -187,7 +200,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
       ral[v] = *(const us8 *)(Alg + o + 8 * v);
     }
 #pragma unroll
-    for (int q = 0; q < NB; q++) rb[q] = load_b2<float>(gb, it, k0 + bk + q, n0 + 2 * bp);
+    for (int q = 0; q < NB; q++) rb[q] = load_b2<float>(gb, it, k0 + bk + q, n0 + 2 * bp, bcx);
   };
   f4 acc[2][2], cor[2][2];
 #pragma unroll
@@ -282,6 +295,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const int bp = tid & 63, bk = (tid >> 6) * 8;
   us8 rah[2], ral[2];
   pair_t<float> rb[8];
+  const BColIdx bcx = bcols(gb, it, n0 + 2 * bp);
   auto gload = [&](int k0) {
     const int64_t o = (int64_t)(m0 + ar) * gb.lda + k0 + ak;
 #pragma unroll
@@ -290,7 +304,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       ral[v] = arow ? *(const us8 *)(Alg + o + 8 * v) : (us8){0, 0, 0, 0, 0, 0, 0, 0};
     }
 #pragma unroll
-    for (int q = 0; q < 8; q++) rb[q] = load_b2<float>(gb, it, k0 + bk + q, n0 + 2 * bp);
+    for (int q = 0; q < 8; q++) rb[q] = load_b2<float>(gb, it, k0 + bk + q, n0 + 2 * bp, bcx);
   };
   f4 acc[4][4], cor[4][4];
 #pragma unroll

@@ -324,6 +324,44 @@ def test_sort_from_keys_widths(env, gi):
         assert int(nps.item()) == ref_off[nbins - 1], (case, Nx, Nr)
 
 
+@pytest.mark.parametrize("n", [16_777_316, 33_563_401])
+def test_sort_from_keys_large(env, n):
+    """chm_sort_from_keys at 16.8M and 33.6M keys (thousands of 4096-key
+    tiles, a partial last tile): 21-bit keys of a 4096 x 512 grid, both
+    uniform and nearly sorted (cell order with local shuffles and a few far
+    moves, as after a step of a cell-ordered plasma).  sort_indx is the
+    stable argsort, cell_offset and Np_stay as the reference defines them."""
+    import ctypes
+    from chimeracl_amd import _lib
+    Nx, Nr = 4096, 512
+    nbins = (Nx - 1) * (Nr - 1) + 1
+    g = _lib.Grid(Nx, Nr, 1, 0, 0.0, 1.0, 0.0, 1.0)
+    rng = np.random.default_rng(n)
+    dev = torch.device("cuda")
+    near = np.sort(rng.integers(0, nbins - 1, n)).astype(np.int64)
+    near += rng.integers(-1, 2, n)
+    far = rng.random(n) < 1e-3
+    near[far] = rng.integers(0, nbins, int(far.sum()))
+    for case, key in (("uniform", rng.integers(0, nbins, n)), ("near-sorted", np.clip(near, 0, nbins - 1))):
+        key = key.astype(np.uint32)
+        print(f"large sort n={n} {case}", flush=True)
+        kd = torch.from_numpy(key.view(np.int32)).to(dev)
+        off = torch.full((nbins + 1,), -1, dtype=torch.int32, device=dev)
+        srt = torch.full((n,), -1, dtype=torch.int32, device=dev)
+        nps = torch.full((1,), -1, dtype=torch.int32, device=dev)
+        ws = torch.empty(int(_lib.lib.chm_sort_workspace_bytes(n, nbins)), dtype=torch.uint8, device=dev)
+        _lib.call("chm_sort_from_keys", n, ctypes.byref(g), kd.data_ptr(), off.data_ptr(),
+                  srt.data_ptr(), nps.data_ptr(), ws.data_ptr(), ws.numel(),
+                  torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        ref = np.argsort(key, kind="stable")
+        assert np.array_equal(srt.cpu().numpy().view(np.uint32), ref), case
+        ref_off = np.searchsorted(key[ref], np.arange(nbins + 1), side="left")
+        assert np.array_equal(off.cpu().numpy().view(np.uint32), ref_off), case
+        assert int(nps.item()) == ref_off[nbins - 1], case
+        del kd, off, srt, ws
+
+
 def test_dens_profile_bit_exact(env):
     g = env.tiny_grid()
     solver, p, s, q = _pair_particles(env, g, 5000, 3, 1)
