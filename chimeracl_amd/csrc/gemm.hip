@@ -51,15 +51,32 @@ __device__ __forceinline__ BColIdx bcols(const GemmBatch &gb, const GemmItem &it
   }
   return r;
 }
-template <typename T>
-__device__ __forceinline__ pair_t<T> load_b2(const GemmBatch &gb, const GemmItem &it, int k, int n,
-                                            BColIdx cx) {
-  pair_t<T> r = {(T)0, (T)0};
-  if (k >= gb.K || n >= gb.N) return r;
-  if (gb.bmode == 0) {
+// The B mode is a template parameter of the product kernels.  BMODE 0 (a
+// real field) loads each column pair straight into the values it stores.
+// BMODE 1 (alpha X + beta Y of complex fields) keeps the loaded elements
+// raw (load_b) and forms the B values only where it stores them to LDS
+// (b_value): a block issues the next K-step's loads, multiplies the current
+// one, and only then computes, so the loads stay in flight across the MFMAs
+// (with the complex products formed right after the loads, each K-step
+// waited for them: the mixed rot product took 597 us per step instead of
+// 360 on the 4096 x 512 step).
+template <typename T> struct braw_t {
+  T x0, x1, y0, y1;
+  bool ok;
+};
+template <typename T, int BMODE> struct BRaw { typedef pair_t<T> type; };
+template <typename T> struct BRaw<T, 1> { typedef braw_t<T> type; };
+
+template <typename T, int BMODE>
+__device__ __forceinline__ typename BRaw<T, BMODE>::type load_b(const GemmBatch &gb,
+                                                                const GemmItem &it, int k, int n,
+                                                                BColIdx cx) {
+  typedef T t2 __attribute__((ext_vector_type(2)));
+  if constexpr (BMODE == 0) {
+    pair_t<T> r = {(T)0, (T)0};
+    if (k >= gb.K || n >= gb.N) return r;
     const T *p = (const T *)it.X + (int64_t)k * gb.ldb + n;
     if (n + 1 < gb.N && !(gb.ldb & 1)) {
-      typedef T t2 __attribute__((ext_vector_type(2)));
       const t2 v = *(const t2 *)p;
       r.a = v.x;
       r.b = v.y;
@@ -68,27 +85,47 @@ __device__ __forceinline__ pair_t<T> load_b2(const GemmBatch &gb, const GemmItem
       if (n + 1 < gb.N) r.b = p[1];
     }
     return r;
-  }
-  typedef T t2 __attribute__((ext_vector_type(2)));
-  const int Nx = gb.Nx;
-  {
+  } else {
+    braw_t<T> r = {(T)0, (T)0, (T)0, (T)0, false};
+    if (k >= gb.K || n >= gb.N) return r;
+    r.ok = true;
+    const int Nx = gb.Nx;
     const t2 v = ((const t2 *)it.X)[(int64_t)k * Nx + cx.x];
-    const T a = it.remapX ? -v.x : v.x, b = v.y;  // -conj(F(-kx)): (-re, +im)
-    r.a = (T)it.alpha.x * a - (T)it.alpha.y * b;
-    r.b = (T)it.alpha.x * b + (T)it.alpha.y * a;
+    r.x0 = v.x;
+    r.x1 = v.y;
+    if (it.Y) {
+      const t2 w = ((const t2 *)it.Y)[(int64_t)k * Nx + cx.y];
+      r.y0 = w.x;
+      r.y1 = w.y;
+    }
+    return r;
   }
-  if (it.Y) {
-    const t2 v = ((const t2 *)it.Y)[(int64_t)k * Nx + cx.y];
-    const T a = it.remapY ? -v.x : v.x, b = v.y;
-    r.a += (T)it.beta.x * a - (T)it.beta.y * b;
-    r.b += (T)it.beta.x * b + (T)it.beta.y * a;
+}
+template <typename T, int BMODE>
+__device__ __forceinline__ pair_t<T> b_value(const GemmItem &it,
+                                             const typename BRaw<T, BMODE>::type &v) {
+  if constexpr (BMODE == 0) {
+    return v;
+  } else {
+    pair_t<T> r = {(T)0, (T)0};
+    if (!v.ok) return r;
+    {
+      const T a = it.remapX ? -v.x0 : v.x0, b = v.x1;  // -conj(F(-kx)): (-re, +im)
+      r.a = (T)it.alpha.x * a - (T)it.alpha.y * b;
+      r.b = (T)it.alpha.x * b + (T)it.alpha.y * a;
+    }
+    if (it.Y) {
+      const T a = it.remapY ? -v.y0 : v.y0, b = v.y1;
+      r.a += (T)it.beta.x * a - (T)it.beta.y * b;
+      r.b += (T)it.beta.x * b + (T)it.beta.y * a;
+    }
+    return r;
   }
-  return r;
 }
 
 // ---------------------------------------------------------------- f64 / f32
 // (registers held to 5 waves per SIMD: the accumulators stay in VGPRs)
-template <typename T>
+template <typename T, int BMODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_gemm(GemmBatch gb) {
   constexpr int BK = 16;
   constexpr int AP = (sizeof(T) == 8) ? 8 : 16;  // row pad: stride = 128 B mod 256 B
@@ -104,14 +141,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
   // B: thread owns columns 2*bp, 2*bp+1 and rows bk, bk+8
   const int bp = tid & 31, bk = tid >> 5;
   T ra[4];
-  pair_t<T> rb[2];
+  typename BRaw<T, BMODE>::type rb[2];
   const BColIdx bcx = bcols(gb, it, n0 + 2 * bp);
   auto gload = [&](int k0) {
     const T *pa = A + (int64_t)(m0 + ar) * gb.lda + k0 + ak;
 #pragma unroll
     for (int q = 0; q < 4; q++) ra[q] = pa[q];
 #pragma unroll
-    for (int q = 0; q < 2; q++) rb[q] = load_b2<T>(gb, it, k0 + bk + 8 * q, n0 + 2 * bp, bcx);
+    for (int q = 0; q < 2; q++) rb[q] = load_b<T, BMODE>(gb, it, k0 + bk + 8 * q, n0 + 2 * bp, bcx);
   };
   typedef typename std::conditional<sizeof(T) == 8, d4, f4>::type acc_t;
   acc_t acc[2][2];
@@ -125,8 +162,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
     for (int q = 0; q < 4; q++) As[ak + q][ar] = ra[q];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
-      Bs[bk + 8 * q][2 * bp] = rb[q].a;
-      Bs[bk + 8 * q][2 * bp + 1] = rb[q].b;
+      const pair_t<T> b = b_value<T, BMODE>(it, rb[q]);
+      Bs[bk + 8 * q][2 * bp] = b.a;
+      Bs[bk + 8 * q][2 * bp + 1] = b.b;
     }
     __syncthreads();
     if (k0 + BK < gb.K) gload(k0 + BK);
@@ -172,7 +210,7 @@ __device__ __forceinline__ void split_bf16(float v, unsigned short &hi, unsigned
   lo = __builtin_bit_cast(unsigned short, l);
 }
 
-template <int BK>
+template <int BK, int BMODE>
 __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
   constexpr int KP = BK + 8;  // row stride 16-byte aligned, staggers banks
   constexpr int NA = BK / 32;  // 16-byte A vectors per thread (hi and lo)
@@ -190,7 +228,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
   // B: thread owns columns 2*bp, 2*bp+1 and k = bk .. bk + NB - 1
   const int bp = tid & 31, bk = (tid >> 5) * NB;
   us8 rah[NA], ral[NA];
-  pair_t<float> rb[NB];
+  typename BRaw<float, BMODE>::type rb[NB];
   const BColIdx bcx = bcols(gb, it, n0 + 2 * bp);
   auto gload = [&](int k0) {
     const int64_t o = (int64_t)(m0 + ar) * gb.lda + k0 + ak;
@@ -200,7 +238,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
       ral[v] = *(const us8 *)(Alg + o + 8 * v);
     }
 #pragma unroll
-    for (int q = 0; q < NB; q++) rb[q] = load_b2<float>(gb, it, k0 + bk + q, n0 + 2 * bp, bcx);
+    for (int q = 0; q < NB; q++) rb[q] = load_b<float, BMODE>(gb, it, k0 + bk + q, n0 + 2 * bp, bcx);
   };
   f4 acc[2][2], cor[2][2];
 #pragma unroll
@@ -219,11 +257,12 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
       us4 bh0, bl0, bh1, bl1;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
+        const pair_t<float> b = b_value<float, BMODE>(it, rb[q0 + q]);
         unsigned short h, l;
-        split_bf16(rb[q0 + q].a, h, l);
+        split_bf16(b.a, h, l);
         bh0[q] = h;
         bl0[q] = l;
-        split_bf16(rb[q0 + q].b, h, l);
+        split_bf16(b.b, h, l);
         bh1[q] = h;
         bl1[q] = l;
       }
@@ -278,6 +317,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16x3(GemmBatch gb) {
 // 64 x 64 quadrant of 4 x 4 MFMA tiles), for launches with enough tiles to
 // fill the GPU: four times the matrix-core work per LDS round trip and
 // barrier pair, and half the operand loads per flop.
+template <int BMODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_gemm128_bf16x3(GemmBatch gb) {
   constexpr int TM = 128, TN = 128, BK = 32, KP = BK + 8;
   __shared__ __attribute__((aligned(16))) unsigned short Ah[TM][KP], Al[TM][KP];
@@ -294,7 +334,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   // B: thread owns columns 2*bp, 2*bp+1 and k = bk .. bk + 7
   const int bp = tid & 63, bk = (tid >> 6) * 8;
   us8 rah[2], ral[2];
-  pair_t<float> rb[8];
+  typename BRaw<float, BMODE>::type rb[8];
   const BColIdx bcx = bcols(gb, it, n0 + 2 * bp);
   auto gload = [&](int k0) {
     const int64_t o = (int64_t)(m0 + ar) * gb.lda + k0 + ak;
@@ -304,7 +344,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       ral[v] = arow ? *(const us8 *)(Alg + o + 8 * v) : (us8){0, 0, 0, 0, 0, 0, 0, 0};
     }
 #pragma unroll
-    for (int q = 0; q < 8; q++) rb[q] = load_b2<float>(gb, it, k0 + bk + q, n0 + 2 * bp, bcx);
+    for (int q = 0; q < 8; q++) rb[q] = load_b<float, BMODE>(gb, it, k0 + bk + q, n0 + 2 * bp, bcx);
   };
   f4 acc[4][4], cor[4][4];
 #pragma unroll
@@ -322,11 +362,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       us8 bh0, bl0, bh1, bl1;
 #pragma unroll
       for (int q = 0; q < 8; q++) {
+        const pair_t<float> b = b_value<float, BMODE>(it, rb[q]);
         unsigned short h, l;
-        split_bf16(rb[q].a, h, l);
+        split_bf16(b.a, h, l);
         bh0[q] = h;
         bl0[q] = l;
-        split_bf16(rb[q].b, h, l);
+        split_bf16(b.b, h, l);
         bh1[q] = h;
         bl1[q] = l;
       }
@@ -475,13 +516,19 @@ int chm_gemm_launch(const GemmBatch &gb_in, int nitems, int kind, hipStream_t st
   dim3 grid128((gb.N + 127) / 128, (gb.M + 127) / 128, nitems);
   TimedLaunch tl;
   const bool timed = t_begin(st, &tl);
+  const bool cplx = gb.bmode == 1;
   if (kind == GEMM_F64) {
-    k_gemm<double><<<grid, 256, 0, st>>>(gb);
+    if (cplx) k_gemm<double, 1><<<grid, 256, 0, st>>>(gb);
+    else k_gemm<double, 0><<<grid, 256, 0, st>>>(gb);
   } else if (kind == GEMM_F32) {
-    k_gemm<float><<<grid, 256, 0, st>>>(gb);
+    if (cplx) k_gemm<float, 1><<<grid, 256, 0, st>>>(gb);
+    else k_gemm<float, 0><<<grid, 256, 0, st>>>(gb);
+  } else if (use128) {
+    if (cplx) k_gemm128_bf16x3<1><<<grid128, 256, 0, st>>>(gb);
+    else k_gemm128_bf16x3<0><<<grid128, 256, 0, st>>>(gb);
   } else {
-    if (use128) k_gemm128_bf16x3<<<grid128, 256, 0, st>>>(gb);
-    else k_gemm_bf16x3<CHM_BF16_BK><<<grid, 256, 0, st>>>(gb);
+    if (cplx) k_gemm_bf16x3<CHM_BF16_BK, 1><<<grid, 256, 0, st>>>(gb);
+    else k_gemm_bf16x3<CHM_BF16_BK, 0><<<grid, 256, 0, st>>>(gb);
   }
   if (timed) {
     tl.flops = 2.0 * gb.M * (double)gb.K * gb.N * nitems;
