@@ -87,16 +87,19 @@ _SIGS = {
     "chm_zero_planes_f32": [R(Planes), i32, i32, i32, i32, P],
     "chm_warp_axis_f64": [R(Planes), i32, i32, i32, i32, P],
     "chm_warp_axis_f32": [R(Planes), i32, i32, i32, i32, P],
-    "chm_gather_push_f64": [P, P, P, P, P, P, P, i64, P, P, f64, R(Grid), R(Planes), R(Planes), P],
-    "chm_gather_push_f32": [P, P, P, P, P, P, P, i64, P, P, f32, R(Grid), R(Planes), R(Planes), P],
-    "chm_gather_push_ids_f64": [P, P, P, P, P, P, P, i64, P, P, P, f64, R(Grid), R(Planes),
+    "chm_gather_push_f64": [P, P, P, P, P, P, P, i64, P, P, P, f64, R(Grid), R(Planes), R(Planes),
+                            P],
+    "chm_gather_push_f32": [P, P, P, P, P, P, P, i64, P, P, P, f32, R(Grid), R(Planes), R(Planes),
+                            P],
+    "chm_gather_push_ids_f64": [P, P, P, P, P, P, P, i64, P, P, P, P, f64, R(Grid), R(Planes),
                                 R(Planes), P],
-    "chm_gather_push_ids_f32": [P, P, P, P, P, P, P, i64, P, P, P, f32, R(Grid), R(Planes),
+    "chm_gather_push_ids_f32": [P, P, P, P, P, P, P, i64, P, P, P, P, f32, R(Grid), R(Planes),
                                 R(Planes), P],
-    "chm_gather_push_pre_f64": [P, P, P, P, P, P, P, i64, P, P, P, P, f64, f64, P, P, P, P,
+    "chm_gather_push_pre_f64": [P, P, P, P, P, P, P, i64, P, P, P, P, P, f64, f64, P, P, P, P,
                                 R(Grid), R(Planes), R(Planes), P],
-    "chm_gather_push_pre_f32": [P, P, P, P, P, P, P, i64, P, P, P, P, f32, f32, P, P, P, P,
+    "chm_gather_push_pre_f32": [P, P, P, P, P, P, P, i64, P, P, P, P, P, f32, f32, P, P, P, P,
                                 R(Grid), R(Planes), R(Planes), P],
+    "chm_sort_sorted_keys": ([P, i64, R(Grid)], P),
     "chm_transform_workspace_bytes": ([i32, i32, i32, i32, i32], sz),
     "chm_transform_f64": [R(Spectral), R(Planes), R(Fb), i32, i32, i32, i32, P, sz, P],
     "chm_transform_f32": [R(Spectral), R(Planes), R(Fb), i32, i32, i32, i32, P, sz, P],

@@ -923,14 +923,14 @@ __device__ __forceinline__ void pre_push(const PrePush<T> &pp, uint32_t p, T xp,
   pp.keys[p] = cell_key<T>(xn, yn, zn, Nx - 1, Nr - 1, xmin, dx_inv, rmin, dr_inv);
 }
 
-template <typename T, int M, bool PRE>
+template <typename T, int M, bool PRE, bool SK>
 __global__ void __launch_bounds__(GS_NT) __attribute__((amdgpu_waves_per_eu(GatherCfg<T, M>::wpe(PRE))))
     k_gather_sorted(const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
                     T *__restrict__ px, T *__restrict__ py, T *__restrict__ pz,
                     T *__restrict__ g_inv, const uint32_t *__restrict__ sort_indx,
                     const uint32_t *__restrict__ np_stay_p, const uint32_t *__restrict__ pid,
                     T factor_push, int Nx, int Nr, T xmin, T dx_inv, T rmin, T dr_inv,
-                    FieldPtrs fp, PrePush<T> pp) {
+                    FieldPtrs fp, PrePush<T> pp, const uint32_t *__restrict__ skey) {
   typedef GatherCfg<T, M> Cfg;
   constexpr int NV = Cfg::NV, MAXN = Cfg::MAXN, IT = Cfg::IT;
   __shared__ T fs[2 * MAXN * NV];
@@ -949,6 +949,25 @@ __global__ void __launch_bounds__(GS_NT) __attribute__((amdgpu_waves_per_eu(Gath
     const int64_t slot = s0 + q * GS_NT + threadIdx.x;
     p[q] = slot < np_stay ? sort_indx[slot] : 0u;
   }
+  // The block's cells: with the sorted keys (SK), from the keys of its
+  // first and last slot (key = ix + ir (Nx - 1) for the valid cells below
+  // Np_stay), so that the field staging loads go out together with the
+  // particle loads; otherwise from the first and last particle's position.
+  // The two node rows under the cells are staged when the cells lie in one
+  // row and span at most MAXN - 1 of them.
+  int ixA = 0, irA = 0, ncol = 0;
+  bool staged = false;
+  if constexpr (SK) {
+    if (threadIdx.x == 0) {
+      const uint32_t c0 = skey[s0], c1 = skey[s0 + nlive - 1];
+      const uint32_t nxl = (uint32_t)(Nx - 1);
+      cab[0] = (int)(c0 % nxl);
+      cab[1] = (int)(c0 / nxl);
+      cab[2] = (int)(c1 % nxl);
+      cab[3] = (int)(c1 / nxl);
+    }
+    __syncthreads();
+  }
 #pragma unroll
   for (int q = 0; q < IT; q++) {
     xp[q] = yp[q] = zp[q] = rp[q] = 0;
@@ -963,6 +982,20 @@ __global__ void __launch_bounds__(GS_NT) __attribute__((amdgpu_waves_per_eu(Gath
       u[q][2] = pz[p[q]];
     }
   }
+  if constexpr (SK) {
+    // (the range is also checked against the grid: keys that no longer
+    // match the positions cannot make the staging read outside it)
+    ixA = cab[0];
+    irA = cab[1];
+    ncol = cab[2] - ixA + 2;
+    staged = cab[3] == irA && ncol >= 2 && ncol <= MAXN && irA >= 0 && irA + 1 < Nr &&
+             ixA >= 0 && ixA + ncol <= Nx;
+    if (staged && (int)threadIdx.x < 2 * ncol) {
+      const int i = (int)threadIdx.x >= ncol ? 1 : 0;
+      const int64_t node = (int64_t)(ixA + (int)threadIdx.x - i * ncol) + (int64_t)(irA + i) * Nx;
+      stage_node<T, M>(fp, node, fs + threadIdx.x * NV);
+    }
+  }
 #pragma unroll
   for (int q = 0; q < IT; q++) {
     if (s0 + q * GS_NT + threadIdx.x < np_stay) {
@@ -971,26 +1004,29 @@ __global__ void __launch_bounds__(GS_NT) __attribute__((amdgpu_waves_per_eu(Gath
       ir[q] = (int)dfloor<T>((rp[q] - rmin) * dr_inv);
     }
   }
-  // slots below Np_stay hold particles of valid cells, in cell order: the
-  // first and the last live slot bound the block's cells
-  if (threadIdx.x == 0) {
-    cab[0] = ix[0];
-    cab[1] = ir[0];
-  }
-#pragma unroll
-  for (int q = 0; q < IT; q++)
-    if (q * GS_NT + (int)threadIdx.x == nlive - 1) {
-      cab[2] = ix[q];
-      cab[3] = ir[q];
+  if constexpr (!SK) {
+    // slots below Np_stay hold particles of valid cells, in cell order: the
+    // first and the last live slot bound the block's cells
+    if (threadIdx.x == 0) {
+      cab[0] = ix[0];
+      cab[1] = ir[0];
     }
-  __syncthreads();
-  const int ixA = cab[0], irA = cab[1];
-  const int ncol = cab[2] - ixA + 2;  // node columns under the block's cells
-  const bool staged = cab[3] == irA && ncol <= MAXN;
-  if (staged && (int)threadIdx.x < 2 * ncol) {
-    const int i = (int)threadIdx.x >= ncol ? 1 : 0;
-    const int64_t node = (int64_t)(ixA + (int)threadIdx.x - i * ncol) + (int64_t)(irA + i) * Nx;
-    stage_node<T, M>(fp, node, fs + threadIdx.x * NV);
+#pragma unroll
+    for (int q = 0; q < IT; q++)
+      if (q * GS_NT + (int)threadIdx.x == nlive - 1) {
+        cab[2] = ix[q];
+        cab[3] = ir[q];
+      }
+    __syncthreads();
+    ixA = cab[0];
+    irA = cab[1];
+    ncol = cab[2] - ixA + 2;  // node columns under the block's cells
+    staged = cab[3] == irA && ncol <= MAXN;
+    if (staged && (int)threadIdx.x < 2 * ncol) {
+      const int i = (int)threadIdx.x >= ncol ? 1 : 0;
+      const int64_t node = (int64_t)(ixA + (int)threadIdx.x - i * ncol) + (int64_t)(irA + i) * Nx;
+      stage_node<T, M>(fp, node, fs + threadIdx.x * NV);
+    }
   }
   __syncthreads();
   const T dt_2 = (T)0.5 * factor_push;
@@ -1026,7 +1062,11 @@ __global__ void __launch_bounds__(GS_NT) __attribute__((amdgpu_waves_per_eu(Gath
     C[1][0] = sR1 * sX0;
     C[1][1] = sR1 * sX1;
     T e_p[3] = {0, 0, 0}, b_p[3] = {0, 0, 0};
-    if (staged) {
+    // (with the keys, a particle outside the staged cells, only possible if
+    // the positions moved after the sort, reads the planes directly)
+    bool from_lds = staged;
+    if constexpr (SK) from_lds = staged && ir[q] == irA && lc >= 0 && lc <= ncol - 2;
+    if (from_lds) {
       const T *f0 = fs + lc * NV;
       gather_sum<T, M, true>([&](int v, int i, int j) { return f0[(i * ncol + j) * NV + v]; }, C,
                              gr, gi, e_p, b_p);
@@ -1050,7 +1090,7 @@ static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, 
                        int64_t Np, const uint32_t *sort_indx, const uint32_t *np_stay,
                        const uint32_t *pid, T factor_push, const chm_grid_t *g,
                        const chm_planes_t *E, const chm_planes_t *B, void *stream,
-                       const uint32_t *cell_offset = nullptr,
+                       const uint32_t *skey, const uint32_t *cell_offset = nullptr,
                        PrePush<T> pp = PrePush<T>{(T)0, nullptr, nullptr, nullptr, nullptr}) {
   const bool pre = pp.keys != nullptr;
   if (pre && (!sort_indx || !cell_offset || !pp.x || !pp.y || !pp.z)) return CHM_EINVAL;
@@ -1067,17 +1107,26 @@ static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, 
   }
   hipStream_t st = (hipStream_t)stream;
   unsigned nb = chm_blocks(Np, 256);
+  // The sorted keys are used in FP64 only: the FP32 kernel with the key path
+  // needs more registers than its bound allows (spills) and measured slower
+  // at 4096 x 512, M = 2 (1.30 against 1.13 ms per call; FP64 2.08 against
+  // 2.17)
+  const bool sk = skey != nullptr && sizeof(T) == 8;
+#define GATHER_SORTED(MM, PREV, SKV)                                                               \
+  k_gather_sorted<T, MM, PREV, SKV><<<chm_blocks(Np, GS_NT * GatherCfg<T, MM>::IT), GS_NT, 0, st>>>( \
+      x, y, z, px, py, pz, g_inv, sort_indx, np_stay, pid, factor_push, g->Nx, g->Nr, (T)g->xmin,  \
+      (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp, pp, skey)
   switch (g->M) {
 #define CASE(MM)                                                                                   \
   case MM:                                                                                         \
-    if (pre)                                                                                       \
-      k_gather_sorted<T, MM, true><<<chm_blocks(Np, GS_NT * GatherCfg<T, MM>::IT), GS_NT, 0, st>>>( \
-          x, y, z, px, py, pz, g_inv, sort_indx, np_stay, pid, factor_push, g->Nx, g->Nr,          \
-          (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp, pp);                             \
+    if (pre && sk)                                                                                 \
+      GATHER_SORTED(MM, true, true);                                                               \
+    else if (pre)                                                                                  \
+      GATHER_SORTED(MM, true, false);                                                              \
+    else if (sort_indx && sk)                                                                      \
+      GATHER_SORTED(MM, false, true);                                                              \
     else if (sort_indx)                                                                            \
-      k_gather_sorted<T, MM, false><<<chm_blocks(Np, GS_NT * GatherCfg<T, MM>::IT), GS_NT, 0, st>>>( \
-          x, y, z, px, py, pz, g_inv, sort_indx, np_stay, pid, factor_push, g->Nx, g->Nr,          \
-          (T)g->xmin, (T)g->dx_inv, (T)g->rmin, (T)g->dr_inv, fp, pp);                             \
+      GATHER_SORTED(MM, false, false);                                                             \
     else                                                                                           \
       k_gather_push<T, MM><<<nb, 256, 0, st>>>(x, y, z, px, py, pz, g_inv, Np, np_stay, pid,       \
                                                factor_push, g->Nx, g->Nr, (T)g->xmin,              \
@@ -1085,6 +1134,7 @@ static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, 
     break;
     CASE(0) CASE(1) CASE(2) CASE(3) CASE(4)
 #undef CASE
+#undef GATHER_SORTED
   }
   if (pre) {
     // the particles of the out-of-domain bin, which the gather does not visit
@@ -1099,62 +1149,65 @@ static int gather_push(const T *x, const T *y, const T *z, T *px, T *py, T *pz, 
 extern "C" int chm_gather_push_f64(const double *x, const double *y, const double *z, double *px,
                                    double *py, double *pz, double *g_inv, int64_t Np,
                                    const uint32_t *sort_indx, const uint32_t *np_stay,
-                                   double factor_push, const chm_grid_t *g, const chm_planes_t *E,
+                                   const uint32_t *sorted_keys, double factor_push,
+                                   const chm_grid_t *g, const chm_planes_t *E,
                                    const chm_planes_t *B, void *stream) {
   return gather_push<double>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, nullptr,
-                             factor_push, g, E, B, stream);
+                             factor_push, g, E, B, stream, sorted_keys);
 }
 extern "C" int chm_gather_push_f32(const float *x, const float *y, const float *z, float *px,
                                    float *py, float *pz, float *g_inv, int64_t Np,
                                    const uint32_t *sort_indx, const uint32_t *np_stay,
-                                   float factor_push, const chm_grid_t *g, const chm_planes_t *E,
+                                   const uint32_t *sorted_keys, float factor_push,
+                                   const chm_grid_t *g, const chm_planes_t *E,
                                    const chm_planes_t *B, void *stream) {
   return gather_push<float>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, nullptr,
-                            factor_push, g, E, B, stream);
+                            factor_push, g, E, B, stream, sorted_keys);
 }
 extern "C" int chm_gather_push_ids_f64(const double *x, const double *y, const double *z,
                                        double *px, double *py, double *pz, double *g_inv,
                                        int64_t Np, const uint32_t *sort_indx,
                                        const uint32_t *np_stay, const uint32_t *pid,
-                                       double factor_push, const chm_grid_t *g,
-                                       const chm_planes_t *E, const chm_planes_t *B,
-                                       void *stream) {
+                                       const uint32_t *sorted_keys, double factor_push,
+                                       const chm_grid_t *g, const chm_planes_t *E,
+                                       const chm_planes_t *B, void *stream) {
   return gather_push<double>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, pid, factor_push,
-                             g, E, B, stream);
+                             g, E, B, stream, sorted_keys);
 }
 extern "C" int chm_gather_push_ids_f32(const float *x, const float *y, const float *z, float *px,
                                        float *py, float *pz, float *g_inv, int64_t Np,
                                        const uint32_t *sort_indx, const uint32_t *np_stay,
-                                       const uint32_t *pid, float factor_push,
-                                       const chm_grid_t *g, const chm_planes_t *E,
-                                       const chm_planes_t *B, void *stream) {
+                                       const uint32_t *pid, const uint32_t *sorted_keys,
+                                       float factor_push, const chm_grid_t *g,
+                                       const chm_planes_t *E, const chm_planes_t *B,
+                                       void *stream) {
   return gather_push<float>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, pid, factor_push,
-                            g, E, B, stream);
+                            g, E, B, stream, sorted_keys);
 }
 extern "C" int chm_gather_push_pre_f64(const double *x, const double *y, const double *z,
                                        double *px, double *py, double *pz, double *g_inv,
                                        int64_t Np, const uint32_t *sort_indx,
                                        const uint32_t *cell_offset, const uint32_t *np_stay,
-                                       const uint32_t *pid, double factor_push, double dt,
-                                       double *xh, double *yh, double *zh,
-                                       uint32_t *indx_in_cell, const chm_grid_t *g,
+                                       const uint32_t *pid, const uint32_t *sorted_keys,
+                                       double factor_push, double dt, double *xh, double *yh,
+                                       double *zh, uint32_t *indx_in_cell, const chm_grid_t *g,
                                        const chm_planes_t *E, const chm_planes_t *B,
                                        void *stream) {
   if (!indx_in_cell) return CHM_EINVAL;
   return gather_push<double>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, pid, factor_push,
-                             g, E, B, stream, cell_offset,
+                             g, E, B, stream, sorted_keys, cell_offset,
                              PrePush<double>{dt, xh, yh, zh, indx_in_cell});
 }
 extern "C" int chm_gather_push_pre_f32(const float *x, const float *y, const float *z, float *px,
                                        float *py, float *pz, float *g_inv, int64_t Np,
                                        const uint32_t *sort_indx, const uint32_t *cell_offset,
                                        const uint32_t *np_stay, const uint32_t *pid,
-                                       float factor_push, float dt, float *xh, float *yh,
-                                       float *zh, uint32_t *indx_in_cell, const chm_grid_t *g,
-                                       const chm_planes_t *E, const chm_planes_t *B,
-                                       void *stream) {
+                                       const uint32_t *sorted_keys, float factor_push, float dt,
+                                       float *xh, float *yh, float *zh, uint32_t *indx_in_cell,
+                                       const chm_grid_t *g, const chm_planes_t *E,
+                                       const chm_planes_t *B, void *stream) {
   if (!indx_in_cell) return CHM_EINVAL;
   return gather_push<float>(x, y, z, px, py, pz, g_inv, Np, sort_indx, np_stay, pid, factor_push,
-                            g, E, B, stream, cell_offset,
+                            g, E, B, stream, sorted_keys, cell_offset,
                             PrePush<float>{dt, xh, yh, zh, indx_in_cell});
 }

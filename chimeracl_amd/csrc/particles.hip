@@ -691,6 +691,16 @@ extern "C" int chm_push_x_keys_f32(float *x, float *y, float *z, const float *px
   return sort_keys<float, true>(x, y, z, px, py, pz, g_inv, dt, Np, g, indx_in_cell, ws, ws_bytes,
                                 stream);
 }
+// Where the last pass left the sorted keys in a sort workspace: the second
+// key buffer after an even number of passes, the first after an odd number.
+extern "C" const uint32_t *chm_sort_sorted_keys(const void *ws, int64_t Np, const chm_grid_t *g) {
+  if (!ws || !g || Np <= 0 || Np > 0x7FFFFFF0ll || g->Nx < 4 || g->Nr < 3) return nullptr;
+  const int64_t nbins = (int64_t)(g->Nx - 1) * (g->Nr - 1) + 1;
+  SortWs w;
+  sort_layout(Np, nbins, &w, (void *)ws);
+  return (rs_plan(key_bits(nbins)).npass & 1) ? w.kA : w.kB;
+}
+
 extern "C" int chm_sort_from_keys(int64_t Np, const chm_grid_t *g, const uint32_t *indx_in_cell,
                                   uint32_t *cell_offset, uint32_t *sort_indx, uint32_t *np_stay,
                                   void *ws, size_t ws_bytes, void *stream) {

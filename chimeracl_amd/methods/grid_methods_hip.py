@@ -166,20 +166,21 @@ class GridMethodsHIP(GenericMethodsHIP):
         comps = self.Args["vec_comps"]
         E = self._planes_arr([flds[0] + c for c in comps])
         B = self._planes_arr([flds[1] + c for c in comps])
+        skey = parts.sorted_keys_ptr(self)
         if prepush:
             parts.ensure_prepush_buffers(self)
             self.call("chm_gather_push_pre", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]),
                       ptr(d["py"]), ptr(d["pz"]), ptr(d["g_inv"]), parts.Args["Np"],
                       ptr(d["sort_indx"]), ptr(d["cell_offset"]), ptr(d["Np_stay"]),
-                      ptr(parts.gather_skip_ids()), parts.Args["FactorPush"], parts.Args["dt_2"],
-                      ptr(d["xh"]), ptr(d["yh"]), ptr(d["zh"]), ptr(d["indx_in_cell"]),
-                      self._grid(), E, B)
+                      ptr(parts.gather_skip_ids()), skey, parts.Args["FactorPush"],
+                      parts.Args["dt_2"], ptr(d["xh"]), ptr(d["yh"]), ptr(d["zh"]),
+                      ptr(d["indx_in_cell"]), self._grid(), E, B)
             return
         args = (ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]), ptr(d["py"]), ptr(d["pz"]),
                 ptr(d["g_inv"]), parts.Args["Np"], ptr(d["sort_indx"]), ptr(d["Np_stay"]))
         ids = parts.gather_skip_ids()
         if ids is not None:  # reordered storage (or the "cell" skip rule)
-            self.call("chm_gather_push_ids", *args, ptr(ids), parts.Args["FactorPush"],
+            self.call("chm_gather_push_ids", *args, ptr(ids), skey, parts.Args["FactorPush"],
                       self._grid(), E, B)
         else:
-            self.call("chm_gather_push", *args, parts.Args["FactorPush"], self._grid(), E, B)
+            self.call("chm_gather_push", *args, skey, parts.Args["FactorPush"], self._grid(), E, B)

@@ -204,9 +204,22 @@ class ParticleMethodsHIP(GenericMethodsHIP):
                       gs, ptr(d["indx_in_cell"]), ptr(d["cell_offset"]), ptr(d["sort_indx"]),
                       ptr(d["Np_stay"]), ptr(ws), ws.numel())
         self._keys_for = None
+        self._sorted_on = grid
         self.version += 1
         dev_np_stay = d["Np_stay"]
         self.Args.set_lazy("Np_stay", lambda: int(dev_np_stay.item()))
+
+    def sorted_keys_ptr(self, grid):
+        """Device address of the keys of the current sort's slots, which the
+        sort left in its workspace (for the gather), or None when the species
+        is not sorted on this grid."""
+        if (not self.flag_sorted or self._sort_ws is None or self.Args["Np"] <= 0
+                or getattr(self, "_sorted_on", None) is not grid):
+            return None
+        key, ws, _ = self._sort_ws
+        if key != (self.Args["Np"], grid.Args["Nxm1Nrm1"] + 1):
+            return None
+        return _lib.lib.chm_sort_sorted_keys(ptr(ws), self.Args["Np"], grid._grid()) or None
 
     # ---------------------------------------------------------------- align
     def align_and_damp(self, comps_align):

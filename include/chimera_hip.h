@@ -205,6 +205,13 @@ int chm_sort_from_keys(int64_t Np, const chm_grid_t *g,
                        const uint32_t *indx_in_cell, uint32_t *cell_offset,
                        uint32_t *sort_indx, uint32_t *np_stay, void *ws,
                        size_t ws_bytes, void *stream);
+/* Device address, inside a sort workspace, of the sorted keys that the last
+ * chm_index_sort_* / chm_sort_from_keys call on it left there (key of each
+ * sorted slot, [Np]; valid until the workspace's next sort).  Host-side
+ * address arithmetic only, nothing is launched; NULL for bad arguments.  No
+ * counterpart in the reference: the gather (chm_gather_push_*) takes it. */
+const uint32_t *chm_sort_sorted_keys(const void *ws, int64_t Np,
+                                     const chm_grid_t *g);
 /* chm_index_sort_f64 without a workspace argument and without allocating:
  * no counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops, harnesses that forbid
@@ -336,19 +343,24 @@ int chm_warp_axis_f32(const chm_planes_t *flds, int32_t nflds, int32_t Nx,
  * positions); the kernel then visits slots ip < *np_stay with
  * p = sort_indx[ip] as the reference does and stages the fields of each
  * slot block's cells in LDS.  NULL: one lane per particle index, fields
- * read directly.  Both give identical results. */
+ * read directly.  Both give identical results.  sorted_keys (may be NULL):
+ * the keys of that sort's slots (chm_sort_sorted_keys); with them each slot
+ * block finds its cells from two keys and loads the fields it stages while
+ * its particles are being loaded, instead of after (identical results). */
 int chm_gather_push_f64(const double *x, const double *y, const double *z,
                         double *px, double *py, double *pz, double *g_inv,
                         int64_t Np, const uint32_t *sort_indx,
-                        const uint32_t *np_stay, double factor_push,
-                        const chm_grid_t *g, const chm_planes_t *E,
-                        const chm_planes_t *B, void *stream);
+                        const uint32_t *np_stay, const uint32_t *sorted_keys,
+                        double factor_push, const chm_grid_t *g,
+                        const chm_planes_t *E, const chm_planes_t *B,
+                        void *stream);
 int chm_gather_push_f32(const float *x, const float *y, const float *z,
                         float *px, float *py, float *pz, float *g_inv,
                         int64_t Np, const uint32_t *sort_indx,
-                        const uint32_t *np_stay, float factor_push,
-                        const chm_grid_t *g, const chm_planes_t *E,
-                        const chm_planes_t *B, void *stream);
+                        const uint32_t *np_stay, const uint32_t *sorted_keys,
+                        float factor_push, const chm_grid_t *g,
+                        const chm_planes_t *E, const chm_planes_t *B,
+                        void *stream);
 /* The same, for storage reordered by chm_align_elems (PIC_loop(align_every)):
  * pid[p] is particle p's index in the numbering without reordering, and the
  * reference's skip (grid_deposit_m1.cl:367, sorting_indx[ip] < Np_stay)
@@ -358,16 +370,16 @@ int chm_gather_push_ids_f64(const double *x, const double *y, const double *z,
                             double *px, double *py, double *pz, double *g_inv,
                             int64_t Np, const uint32_t *sort_indx,
                             const uint32_t *np_stay, const uint32_t *pid,
-                            double factor_push, const chm_grid_t *g,
-                            const chm_planes_t *E, const chm_planes_t *B,
-                            void *stream);
+                            const uint32_t *sorted_keys, double factor_push,
+                            const chm_grid_t *g, const chm_planes_t *E,
+                            const chm_planes_t *B, void *stream);
 int chm_gather_push_ids_f32(const float *x, const float *y, const float *z,
                             float *px, float *py, float *pz, float *g_inv,
                             int64_t Np, const uint32_t *sort_indx,
                             const uint32_t *np_stay, const uint32_t *pid,
-                            float factor_push, const chm_grid_t *g,
-                            const chm_planes_t *E, const chm_planes_t *B,
-                            void *stream);
+                            const uint32_t *sorted_keys, float factor_push,
+                            const chm_grid_t *g, const chm_planes_t *E,
+                            const chm_planes_t *B, void *stream);
 /* chm_gather_push_ids_* (pid may be NULL) followed by the next step's first
  * coordinate push and the keys of its sort (pic_loop.py:134 gather_and_push,
  * then pic_loop.py:70-77 push_coords('half') + the key pass of sort_parts of
@@ -381,20 +393,20 @@ int chm_gather_push_pre_f64(const double *x, const double *y, const double *z,
                             double *px, double *py, double *pz, double *g_inv,
                             int64_t Np, const uint32_t *sort_indx,
                             const uint32_t *cell_offset, const uint32_t *np_stay,
-                            const uint32_t *pid, double factor_push, double dt,
-                            double *xh, double *yh, double *zh,
-                            uint32_t *indx_in_cell, const chm_grid_t *g,
-                            const chm_planes_t *E, const chm_planes_t *B,
-                            void *stream);
+                            const uint32_t *pid, const uint32_t *sorted_keys,
+                            double factor_push, double dt, double *xh,
+                            double *yh, double *zh, uint32_t *indx_in_cell,
+                            const chm_grid_t *g, const chm_planes_t *E,
+                            const chm_planes_t *B, void *stream);
 int chm_gather_push_pre_f32(const float *x, const float *y, const float *z,
                             float *px, float *py, float *pz, float *g_inv,
                             int64_t Np, const uint32_t *sort_indx,
                             const uint32_t *cell_offset, const uint32_t *np_stay,
-                            const uint32_t *pid, float factor_push, float dt,
-                            float *xh, float *yh, float *zh,
-                            uint32_t *indx_in_cell, const chm_grid_t *g,
-                            const chm_planes_t *E, const chm_planes_t *B,
-                            void *stream);
+                            const uint32_t *pid, const uint32_t *sorted_keys,
+                            float factor_push, float dt, float *xh, float *yh,
+                            float *zh, uint32_t *indx_in_cell,
+                            const chm_grid_t *g, const chm_planes_t *E,
+                            const chm_planes_t *B, void *stream);
 
 /* ------------------------------------------------------------- transforms */
 size_t chm_transform_workspace_bytes(int32_t Nx, int32_t Nr, int32_t M,

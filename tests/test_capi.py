@@ -12,7 +12,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     src = open(os.path.join(ROOT, "include", "chimera_hip.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(?:int|int64_t|size_t)\s+(chm_\w+)\s*\(", src)))
+    return sorted(set(re.findall(
+        r"(?:\b(?:int|int64_t|size_t)\s+|\bconst\s+uint32_t\s*\*\s*)(chm_\w+)\s*\(", src)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -84,3 +85,24 @@ def test_dht_timing_record_empty_without_launches():
     assert L.chm_dht_timing(1) == 0
     assert L.chm_dht_timing_collect(None, None, None, 10) == 0
     assert L.chm_dht_timing(0) == 0
+
+
+def test_sort_sorted_keys_address():
+    """chm_sort_sorted_keys is host-side address arithmetic: for a workspace
+    of chm_sort_workspace_bytes(Np, nbins) bytes it names Np keys inside it
+    (4-byte aligned), and NULL for a missing workspace or grid, or Np <= 0.
+    Nothing is launched, so no GPU is needed."""
+    import ctypes
+    from chimeracl_amd import _lib
+    base = 1 << 32
+    for Nx, Nr, Np in ((64, 24, 1000), (4096, 512, 66_797_760), (33, 5, 1)):
+        g = _lib.Grid(Nx, Nr, 1, 0, 0.0, 1.0, 0.0, 1.0)
+        nbins = (Nx - 1) * (Nr - 1) + 1
+        size = _lib.lib.chm_sort_workspace_bytes(Np, nbins)
+        a = _lib.lib.chm_sort_sorted_keys(base, Np, ctypes.byref(g))
+        assert a is not None and a % 4 == 0
+        assert base <= a and a + 4 * Np <= base + size
+    g = _lib.Grid(64, 24, 1, 0, 0.0, 1.0, 0.0, 1.0)
+    assert _lib.lib.chm_sort_sorted_keys(None, 10, ctypes.byref(g)) is None
+    assert _lib.lib.chm_sort_sorted_keys(base, 0, ctypes.byref(g)) is None
+    assert _lib.lib.chm_sort_sorted_keys(base, 10, None) is None
