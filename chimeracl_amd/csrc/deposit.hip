@@ -68,7 +68,12 @@ template <typename T, int M, int KIND> struct DepCfg {
   static constexpr int CELLS = 64 / TPC;               // cells per wave
   // staged slots per chunk (measured at C2)
   static constexpr int CAP = sizeof(T) == 8 ? (VEC ? 192 : 128) : (VEC ? 192 : 256);
-  static constexpr int NV = 4 + 2 * M + (VEC ? 3 : 0); // staged values per particle
+  // mode factors m >= 2 recomputed from the staged m = 1 pair by the summing
+  // lanes, with the staging's operations (identical values), for less LDS
+  // per wave (kind 2 stages doubled factors and keeps them all)
+  static constexpr bool RECOMP = KIND != 2 && M >= 2;
+  static constexpr int NF = RECOMP ? 1 : M;            // staged mode-factor pairs
+  static constexpr int NV = 4 + 2 * NF + (VEC ? 3 : 0); // staged values per particle
 };
 // Lanes walk their own cell's slots, ~ppc apart: one pad element per 32
 // spreads them over the LDS banks.
@@ -86,7 +91,7 @@ __global__ void __launch_bounds__(64)
   constexpr bool VEC = Cfg::VEC;
   constexpr int CAP = Cfg::CAP, NV = Cfg::NV, TPC = Cfg::TPC, CELLS = Cfg::CELLS;
   constexpr int MM = M > 0 ? M : 1;
-  constexpr int VE = 4, VU = 4 + 2 * M;  // first mode-factor / value slot
+  constexpr int VE = 4, VU = 4 + 2 * Cfg::NF;  // first mode-factor / value slot
   constexpr int R = (CAP + 63) / 64;
   constexpr int NQ = 1 + 2 * M;
   __shared__ T st[NV][CAP + CAP / 32];
@@ -193,7 +198,7 @@ __global__ void __launch_bounds__(64)
       st[2][q] = sR1 * sX0;
       st[3][q] = sR1 * sXw1;
 #pragma unroll
-      for (int m = 1; m <= M; m++) {
+      for (int m = 1; m <= Cfg::NF; m++) {
         st[VE + 2 * (m - 1)][q] = er[m];
         st[VE + 2 * (m - 1) + 1][q] = ei[m];
       }
@@ -229,21 +234,33 @@ __global__ void __launch_bounds__(64)
       for (uint32_t ip = lo; ip < hi; ip++) {
         const uint32_t t = DEP_SW(ip - cs);
         const T C = st[corner][t];
+        // the particle's mode factors ((y + i z)/r)^m
+        T fr[MM + 1], fi[MM + 1];
+#pragma unroll
+        for (int m = 1; m <= Cfg::NF; m++) {
+          fr[m] = st[VE + 2 * (m - 1)][t];
+          fi[m] = st[VE + 2 * (m - 1) + 1][t];
+        }
+#pragma unroll
+        for (int m = Cfg::NF + 1; m <= M; m++) {
+          fr[m] = fr[m - 1] * fr[1] - fi[m - 1] * fi[1];
+          fi[m] = fr[m - 1] * fi[1] + fi[m - 1] * fr[1];
+        }
         if (VEC) {
           const T u = st[VU + k][t];
           const T jproj = C * u;
           a0 = madd(C, u, a0);
 #pragma unroll
           for (int m = 1; m <= M; m++) {
-            ar[m - 1] = madd(jproj, st[VE + 2 * (m - 1)][t], ar[m - 1]);
-            ai[m - 1] = madd(jproj, st[VE + 2 * (m - 1) + 1][t], ai[m - 1]);
+            ar[m - 1] = madd(jproj, fr[m], ar[m - 1]);
+            ai[m - 1] = madd(jproj, fi[m], ai[m - 1]);
           }
         } else {
           a0 += C;
 #pragma unroll
           for (int m = 1; m <= M; m++) {
-            ar[m - 1] = madd(C, st[VE + 2 * (m - 1)][t], ar[m - 1]);
-            ai[m - 1] = madd(C, st[VE + 2 * (m - 1) + 1][t], ai[m - 1]);
+            ar[m - 1] = madd(C, fr[m], ar[m - 1]);
+            ai[m - 1] = madd(C, fi[m], ai[m - 1]);
           }
         }
       }
