@@ -141,13 +141,15 @@ def test_gather_push_bit_exact(env, gi):
         assert np.array_equal(env.field_host(solver, name), s.f[name]), name
 
 
-def test_gather_stale_sorted_keys_match_keyless(env):
-    """The FP64 gather given the sort's keys (chm_sort_sorted_keys) after the
-    particles moved by up to 1.5 cells since that sort, so that the keys no
-    longer match the positions: bit-identical to the gather without keys
-    (the staged fields are used only for particles inside the staged cells,
-    the others read the planes).  Moves stay inside the valid cells, where
-    the gather of the sorted slots below Np_stay may read."""
+def test_gather_stale_sorted_keys_match_per_particle(env):
+    """The FP64 gather of the sorted slots given the sort's keys
+    (chm_sort_sorted_keys) after the particles moved by up to 1.5 cells since
+    that sort, so that the keys no longer match the positions: bit-identical
+    to the per-particle gather (sort_indx = NULL), which reads each
+    particle's fields at its own position (the staged fields are used only
+    for particles inside the staged cells, the others read the planes).  All
+    particles are kept inside the valid cells before and after the move, so
+    that both visit every particle."""
     import ctypes
     from chimeracl_amd import _lib
     from chimeracl_amd.methods.generic_methods_hip import ptr
@@ -161,35 +163,42 @@ def test_gather_stale_sorted_keys_match_keyless(env):
                 t = solver.DataDev[f"{f}{c}_m{m}"]
                 v = rng.normal(0, 0.5, tuple(t.shape))
                 t.copy_(torch.from_numpy(v).to(t.dtype))
-    p.sort_parts(solver)
-    skey = p.sorted_keys_ptr(solver)
-    assert skey
     d = p.DataDev
     dx, dr, xmin, rmin = a["dx"], a["dr"], a["Xmin"], a["Rmin"]
-    x = d["x"].cpu().numpy() + rng.uniform(-1.5, 1.5, n) * dx
-    x = np.clip(x, xmin + 1.2 * dx, xmin + (a["Nx"] - 2.2) * dx)
-    y, z = d["y"].cpu().numpy(), d["z"].cpu().numpy()
-    r = np.hypot(y, z)
-    rn = np.clip(r + rng.uniform(-1.5, 1.5, n) * dr, 0.05 * dr, rmin + (a["Nr"] - 2.2) * dr)
-    fac = np.where(r > 0, rn / np.where(r > 0, r, 1.0), 1.0)
-    for k, v in (("x", x), ("y", y * fac), ("z", z * fac)):
-        d[k].copy_(torch.from_numpy(v))
+    xlo, xhi = xmin + 1.2 * dx, xmin + (a["Nx"] - 2.2) * dx
+    rlo, rhi = 0.05 * dr, rmin + (a["Nr"] - 2.2) * dr
+
+    def place(shift):
+        x = np.clip(d["x"].cpu().numpy() + shift * rng.uniform(-1.5, 1.5, n) * dx, xlo, xhi)
+        y, z = d["y"].cpu().numpy(), d["z"].cpu().numpy()
+        r = np.hypot(y, z)
+        rn = np.clip(r + shift * rng.uniform(-1.5, 1.5, n) * dr, rlo, rhi)
+        fac = np.where(r > 0, rn / np.where(r > 0, r, 1.0), 1.0)
+        for k, v in (("x", x), ("y", y * fac), ("z", z * fac)):
+            d[k].copy_(torch.from_numpy(v))
+
+    place(0.0)          # every particle in a valid cell: Np_stay = Np
+    p.flag_sorted = False
+    p.sort_parts(solver)
+    assert p.Args["Np_stay"] == n
+    skey = p.sorted_keys_ptr(solver)
+    assert skey
+    place(1.0)          # move after the sort
     mom = {k: d[k].clone() for k in ("px", "py", "pz", "g_inv")}
     comps = a["vec_comps"]
     E = solver._planes_arr(["E" + c for c in comps])
     B = solver._planes_arr(["B" + c for c in comps])
     out = []
-    for key in (skey, None):
+    for sidx, key in ((ptr(d["sort_indx"]), skey), (None, None)):
         for k, v in mom.items():
             d[k].copy_(v)
-        _lib.call("chm_gather_push_ids_f64", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]),
-                  ptr(d["py"]), ptr(d["pz"]), ptr(d["g_inv"]), n, ptr(d["sort_indx"]),
-                  ptr(d["Np_stay"]), None, key, p.Args["FactorPush"], ctypes.byref(solver._grid()),
-                  E, B, torch.cuda.current_stream().cuda_stream)
+        _lib.call("chm_gather_push_f64", ptr(d["x"]), ptr(d["y"]), ptr(d["z"]), ptr(d["px"]),
+                  ptr(d["py"]), ptr(d["pz"]), ptr(d["g_inv"]), n, sidx, ptr(d["Np_stay"]), key,
+                  p.Args["FactorPush"], ctypes.byref(solver._grid()), E, B,
+                  torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         out.append({k: d[k].cpu().numpy().copy() for k in mom})
-    moved = sum(int(np.any(out[0][k] != mom[k].cpu().numpy())) for k in mom)
-    assert moved > 0   # the gather did push particles
+    assert not np.array_equal(out[0]["px"], mom["px"].cpu().numpy())  # pushed
     for k in mom:
         assert np.array_equal(out[0][k], out[1][k]), k
 
