@@ -27,6 +27,7 @@
 #define RS_BS 256
 #define RS_ITEMS 16
 #define RS_TILE (RS_BS * RS_ITEMS)
+static_assert(RS_TILE <= 65535, "per-tile digit counts are 16-bit");
 
 // ------------------------------------------------------------------ push
 template <typename T>
@@ -313,11 +314,11 @@ __global__ void k_list_clear(uint32_t *__restrict__ gl) { gl[threadIdx.x] = 0u; 
 
 // ------------------------------------------------------------------ radix sort
 // Digits of nbits <= 2^RB buckets; counts[tile * nd + d] (one contiguous
-// row per tile).
+// row per tile), 16 bits: a tile holds RS_TILE = 4096 keys.
 template <int RB>
 __global__ void __launch_bounds__(RS_BS) k_rs_upsweep(const uint32_t *__restrict__ keys, int64_t n,
                                                       int shift, int nbits,
-                                                      uint32_t *__restrict__ counts) {
+                                                      uint16_t *__restrict__ counts) {
   constexpr int NB = 1 << RB;
   __shared__ uint32_t h[NB];
   const int nd = 1 << nbits;
@@ -339,7 +340,7 @@ __global__ void __launch_bounds__(RS_BS) k_rs_upsweep(const uint32_t *__restrict
     run_add(h, act, (kk[i] >> shift) & mask, lane);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < nd; d += RS_BS) counts[(int64_t)blockIdx.x * nd + d] = h[d];
+  for (int d = threadIdx.x; d < nd; d += RS_BS) counts[(int64_t)blockIdx.x * nd + d] = (uint16_t)h[d];
 }
 
 // Lanes of the wave holding the same digit d (active lanes only).  Fast
@@ -463,7 +464,7 @@ __global__ void __launch_bounds__(RS_BS) __attribute__((amdgpu_waves_per_eu(4)))
 // chunk sums transposed (csumT[d * nch + ch]), a flat scan of csumT gives each
 // chunk's starting offset, and k_rs_coloffs runs through the chunk's tiles.
 #define RS_TC 16
-__global__ void __launch_bounds__(256) k_rs_colsum(const uint32_t *__restrict__ counts, int64_t nblk,
+__global__ void __launch_bounds__(256) k_rs_colsum(const uint16_t *__restrict__ counts, int64_t nblk,
                                                    int nbits, int64_t nch,
                                                    uint32_t *__restrict__ csumT,
                                                    uint32_t *__restrict__ clear) {
@@ -483,7 +484,7 @@ __global__ void __launch_bounds__(256) k_rs_colsum(const uint32_t *__restrict__ 
   csumT[(int64_t)d * nch + ch] = s;
 }
 
-__global__ void __launch_bounds__(256) k_rs_coloffs(const uint32_t *__restrict__ counts, int64_t nblk,
+__global__ void __launch_bounds__(256) k_rs_coloffs(const uint16_t *__restrict__ counts, int64_t nblk,
                                                     int nbits, int64_t nch,
                                                     const uint32_t *__restrict__ cpre,
                                                     uint32_t *__restrict__ offs) {
@@ -535,7 +536,8 @@ static int key_bits(int64_t nbins) {
 
 // ------------------------------------------------------------------ driver
 struct SortWs {
-  uint32_t *counts, *offs, *csum, *cpre, *bsum2, *kA, *kB, *vA, *vB, *trunc;
+  uint16_t *counts;
+  uint32_t *offs, *csum, *cpre, *bsum2, *kA, *kB, *vA, *vB, *trunc;
   int64_t nblk, nch;
 };
 
@@ -547,7 +549,7 @@ static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
   const int64_t ncounts = nd * nblk, ncsum = nd * nch;
   const int64_t nscan2 = (ncsum + SCAN_TILE - 1) / SCAN_TILE;
   size_t off = 0;
-  size_t o_counts = off; off += chm_align(ncounts * 4);
+  size_t o_counts = off; off += chm_align(ncounts * 2);
   size_t o_offs = off; off += chm_align(ncounts * 4);
   size_t o_csum = off; off += chm_align(ncsum * 4);
   size_t o_cpre = off; off += chm_align((ncsum + 1) * 4);
@@ -557,7 +559,7 @@ static size_t sort_layout(int64_t Np, int64_t nbins, SortWs *w, void *base) {
   size_t o_t = off; off += chm_align((2 + 3 * OFF_LIST) * 4);
   if (w) {
     char *b = (char *)base;
-    w->counts = (uint32_t *)(b + o_counts);
+    w->counts = (uint16_t *)(b + o_counts);
     w->offs = (uint32_t *)(b + o_offs);
     w->csum = (uint32_t *)(b + o_csum);
     w->cpre = (uint32_t *)(b + o_cpre);
