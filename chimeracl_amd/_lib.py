@@ -68,6 +68,7 @@ _SIGS = {
     "chm_sort_from_keys": [i64, R(Grid), P, P, P, P, P, sz, P],
     "chm_index_sort_f64_noalloc": [P, P, P, i64, R(Grid), P, P, P, P, P],
     "chm_align_elems": [P, i64, P, P, i32, P],
+    "chm_align_multi": [P, i64, i32, P, P, P, P],
     "chm_dens_profile_f64": [P, P, i64, P, P, P, i32, P],
     "chm_dens_profile_f32": [P, P, i64, P, P, P, i32, P],
     "chm_deposit_workspace_bytes": ([i32, i32, i32, i32], sz),

@@ -95,6 +95,59 @@ extern "C" int chm_align_elems(const uint32_t *sort_indx, int64_t n, const void 
   return CHM_OK;
 }
 
+// Several arrays permuted by one launch: the slot's index is read once and
+// the loads of all arrays are in flight together.
+struct AlignSet {
+  const void *src[CHM_ALIGN_MAXARR];
+  void *dst[CHM_ALIGN_MAXARR];
+  int eb[CHM_ALIGN_MAXARR];
+  int narr;
+};
+
+__global__ void __launch_bounds__(256) k_align_multi(const uint32_t *__restrict__ idx, int64_t n,
+                                                     AlignSet as) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = idx[i];
+  uint64_t v[CHM_ALIGN_MAXARR];
+#pragma unroll
+  for (int a = 0; a < CHM_ALIGN_MAXARR; a++)
+    if (a < as.narr)
+      v[a] = as.eb[a] == 8 ? ((const uint64_t *)as.src[a])[p]
+                           : (uint64_t)((const uint32_t *)as.src[a])[p];
+#pragma unroll
+  for (int a = 0; a < CHM_ALIGN_MAXARR; a++)
+    if (a < as.narr) {
+      if (as.eb[a] == 8) ((uint64_t *)as.dst[a])[i] = v[a];
+      else ((uint32_t *)as.dst[a])[i] = (uint32_t)v[a];
+    }
+}
+
+extern "C" int chm_align_multi(const uint32_t *sort_indx, int64_t n, int32_t narr,
+                               const void *const *src, void *const *dst,
+                               const int32_t *elem_bytes, void *stream) {
+  if (n < 0 || narr < 1 || narr > CHM_ALIGN_MAXARR || !src || !dst || !elem_bytes)
+    return CHM_EINVAL;
+  AlignSet as = {};
+  as.narr = narr;
+  for (int a = 0; a < narr; a++) {
+    if ((elem_bytes[a] != 4 && elem_bytes[a] != 8) || (n > 0 && (!src[a] || !dst[a])))
+      return CHM_EINVAL;
+    as.src[a] = src[a];
+    as.dst[a] = dst[a];
+    as.eb[a] = elem_bytes[a];
+  }
+  // no destination may be read by the same launch
+  for (int a = 0; a < narr; a++)
+    for (int b = 0; b < narr; b++)
+      if (n > 0 && (dst[a] == src[b] || (a != b && dst[a] == dst[b]))) return CHM_EINVAL;
+  if (n == 0) return CHM_OK;
+  if (!sort_indx) return CHM_EINVAL;
+  k_align_multi<<<chm_blocks(n, 256), 256, 0, (hipStream_t)stream>>>(sort_indx, n, as);
+  CHM_CHECK_LAUNCH();
+  return CHM_OK;
+}
+
 // ------------------------------------------------------------------ keys
 // Run structure of a wave: lane is a head if its key differs from the lane
 // below.  Returns the head mask; `len` = run length for head lanes.

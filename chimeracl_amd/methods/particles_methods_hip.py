@@ -8,11 +8,15 @@ chimeracl_amd.config.load_plasma.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
 from .. import _lib
 from .generic_methods_hip import GenericMethodsHIP, ptr
+
+ALIGN_MAXARR = 16  # arrays per chm_align_multi launch (CHM_ALIGN_MAXARR)
 
 
 def _grid_struct(grid):
@@ -26,7 +30,8 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         self.set_global_working_group_size()
         self._sort_ws = None
         self._keys_for = None
-        self._align_buf = None
+        self._iota = None
+        self._spare = {}
         # bumped whenever positions, weights, momenta or the order change
         self.version = 0
 
@@ -264,34 +269,47 @@ class ParticleMethodsHIP(GenericMethodsHIP):
         if "pid" not in d and self.Args["Np"] > 0:
             d["pid"] = torch.arange(self.Args["Np"], dtype=torch.int32, device=d["x"].device)
 
-    def reorder_parts(self, extra=()):
+    def reorder_parts(self, extra=(), dead=()):
         """Storage order only, no counterpart in the reference: every
         component of all Np particles is permuted into the current sorted
-        order with the align kernel (none dropped), and sort_indx becomes the
-        identity, so the particles' memory order follows their cells again.
-        Values, cell offsets and Np_stay are unchanged; particle indices
-        change, and DataDev['pid'] keeps each particle's index in the
-        numbering without reordering (the gather's skip of particles with
-        index >= Np_stay uses it, so the same particles are skipped).  extra:
-        further per-particle DataDev arrays to permute along (the pre-pushed
-        positions and their keys).  Needs a current sort (flag_sorted)."""
+        order (none dropped), and sort_indx becomes the identity, so the
+        particles' memory order follows their cells again.  Values, cell
+        offsets and Np_stay are unchanged; particle indices change, and
+        DataDev['pid'] keeps each particle's index in the numbering without
+        reordering (the gather's skip of particles with index >= Np_stay uses
+        it, so the same particles are skipped).  extra: further per-particle
+        DataDev arrays to permute along (the pre-pushed positions and their
+        keys); dead: arrays the step overwrites before it reads them (x, y, z
+        when it consumes pre-pushed positions), left as they are.  Needs a
+        current sort (flag_sorted).  One launch permutes up to 16 arrays into
+        scratch tensors (chm_align_multi, sort_indx read once), which are
+        then copied back: the arrays keep their device addresses, which
+        captured step graphs hold."""
         Np = self.Args["Np"]
         if Np == 0 or not self.flag_sorted:
             return
         d = self.DataDev
         self.ensure_pid()
-        if self._align_buf is None or self._align_buf[0].numel() != Np:
-            dev = d["x"].device
-            self._align_buf = (torch.empty(Np, dtype=torch.float64, device=dev),
-                               torch.arange(Np, dtype=torch.int32, device=dev))
-        scr, iota = self._align_buf
+        if self._iota is None or self._iota.numel() != Np:
+            self._iota = torch.arange(Np, dtype=torch.int32, device=d["x"].device)
+        names = [c for c in self._arg_names() + ["pid"] + list(extra) if c not in dead]
+        sp = self._spare
+        for c in names:
+            t = sp.get(c)
+            if t is None or t.shape != d[c].shape or t.dtype != d[c].dtype:
+                sp[c] = torch.empty_like(d[c])
         srt = d["sort_indx"]
-        for comp in self._arg_names() + ["pid"] + list(extra):
-            a = d[comp]
-            tmp = scr.view(torch.uint8)[: a.numel() * a.element_size()].view(a.dtype)
-            self.call_raw("chm_align_elems", ptr(srt), Np, ptr(a), ptr(tmp), a.element_size())
-            a.copy_(tmp)
-        srt.copy_(iota)
+        for i in range(0, len(names), ALIGN_MAXARR):
+            grp = names[i:i + ALIGN_MAXARR]
+            n = len(grp)
+            src = (ctypes.c_void_p * n)(*(d[c].data_ptr() for c in grp))
+            dst = (ctypes.c_void_p * n)(*(sp[c].data_ptr() for c in grp))
+            eb = (ctypes.c_int32 * n)(*(d[c].element_size() for c in grp))
+            self.call_raw("chm_align_multi", ptr(srt), Np, n, ctypes.addressof(src),
+                          ctypes.addressof(dst), ctypes.addressof(eb))
+        for c in names:
+            d[c].copy_(sp[c])
+        srt.copy_(self._iota)
         if not extra:
             self._keys_for = None
         self.version += 1

@@ -200,8 +200,11 @@ class PIC_loop:
                     # creates it
                     parts.ensure_pid()
                     if self.it > 0 and self.it % self.align_every == 0:
+                        # consuming the pre-push, the step's J deposit
+                        # writes every x, y, z before anything reads them
                         parts.reorder_parts(
-                            ("xh", "yh", "zh", "indx_in_cell") if consume else ())
+                            ("xh", "yh", "zh", "indx_in_cell") if consume else (),
+                            ("x", "y", "z") if consume else ())
 
     def _capture(self, key):
         # A rocprofv3 counter-collection run of the graph-replayed step ended

@@ -237,6 +237,16 @@ int chm_index_sort_f64_noalloc(const double *x, const double *y,
  * uint32).  dst must not be src. */
 int chm_align_elems(const uint32_t *sort_indx, int64_t n, const void *src,
                     void *dst, int32_t elem_bytes, void *stream);
+/* The same permutation of narr (1..CHM_ALIGN_MAXARR) arrays in one launch:
+ * dst[a][i] = src[a][sort_indx[i]] for i < n, elem_bytes[a] 4 or 8; src,
+ * dst and elem_bytes are host arrays of narr entries.  No dst may equal any
+ * src, nor another dst.  The reference aligns one array per call
+ * (particles_methods_cl.py:263-286); this groups the calls so that sort_indx
+ * is read once (PIC_loop(align_every) reorders). */
+#define CHM_ALIGN_MAXARR 16
+int chm_align_multi(const uint32_t *sort_indx, int64_t n, int32_t narr,
+                    const void *const *src, void *const *dst,
+                    const int32_t *elem_bytes, void *stream);
 int chm_dens_profile_f64(const double *x, double *w, int64_t Np,
                          const double *xx, const double *ff,
                          const double *dxm1, int32_t nknots, void *stream);

@@ -141,6 +141,46 @@ def test_gather_push_bit_exact(env, gi):
         assert np.array_equal(env.field_host(solver, name), s.f[name]), name
 
 
+def test_align_multi_matches_numpy(env):
+    """chm_align_multi permutes 8- and 4-byte arrays in one launch,
+    dst[a][i] = src[a][sort_indx[i]], bit for bit as numpy's take and as
+    chm_align_elems per array; a destination that is also a source, or
+    another destination, is refused without a launch."""
+    import ctypes
+    from chimeracl_amd import _lib
+    n = 100_003
+    rng = np.random.default_rng(7)
+    perm = rng.permutation(n).astype(np.uint32)
+    host = [rng.normal(size=n), rng.integers(0, 2**31, n).astype(np.int32),
+            rng.normal(size=n).astype(np.float32), rng.normal(size=n),
+            rng.integers(0, 2**62, n).astype(np.int64)]
+    dev = torch.device("cuda")
+    srt = torch.from_numpy(perm.view(np.int32)).to(dev)
+    src = [torch.from_numpy(h).to(dev) for h in host]
+    dst = [torch.empty_like(t) for t in src]
+    P = ctypes.c_void_p
+    k = len(src)
+    sp = (P * k)(*(t.data_ptr() for t in src))
+    dp = (P * k)(*(t.data_ptr() for t in dst))
+    eb = (ctypes.c_int32 * k)(*(t.element_size() for t in src))
+    st = torch.cuda.current_stream().cuda_stream
+    _lib.call("chm_align_multi", srt.data_ptr(), n, k, ctypes.addressof(sp), ctypes.addressof(dp),
+              ctypes.addressof(eb), st)
+    torch.cuda.synchronize()
+    for h, t in zip(host, dst):
+        assert np.array_equal(t.cpu().numpy(), h[perm])
+    one = torch.empty_like(src[0])
+    _lib.call("chm_align_elems", srt.data_ptr(), n, src[0].data_ptr(), one.data_ptr(), 8, st)
+    torch.cuda.synchronize()
+    assert torch.equal(one, dst[0])
+    bad = (P * 2)(dst[0].data_ptr(), src[0].data_ptr())
+    assert _lib.lib.chm_align_multi(srt.data_ptr(), n, 2, ctypes.addressof(sp), ctypes.addressof(bad),
+                                    ctypes.addressof(eb), st) != 0
+    bad2 = (P * 2)(dst[0].data_ptr(), dst[0].data_ptr())
+    assert _lib.lib.chm_align_multi(srt.data_ptr(), n, 2, ctypes.addressof(sp), ctypes.addressof(bad2),
+                                    ctypes.addressof(eb), st) != 0
+
+
 def test_gather_stale_sorted_keys_match_per_particle(env):
     """The FP64 gather of the sorted slots given the sort's keys
     (chm_sort_sorted_keys) after the particles moved by up to 1.5 cells since
