@@ -74,16 +74,27 @@ __global__ void __launch_bounds__(FFT_NT) k_fft_rows(FftBatch<T> fb) {
 
 // Rows of 4096: the register-resident FFT (fft4096_reg), one row per block
 // of 256 threads, thread t handling elements t + 256 j; same loads, phases,
-// scale, table and stores as k_fft_rows.
-template <typename T, int SIGN>
+// scale, table and stores as k_fft_rows.  VAR >= 0 fixes the plane's form at
+// compile time (bits: 1 complex input, 2 complex output, 4 phase on load,
+// 8 phase on store, 16 table), so that only that form's registers are
+// allocated: in FP64 the form-generic kernel (VAR = -1) holds 200 VGPRs (2
+// waves per SIMD), the backward transform's forms 128 and the forward ones
+// 92 (4 waves, the LDS limit).
+template <typename T, int SIGN, int VAR>
 __global__ void __launch_bounds__(256) k_fft_rows4096(FftBatch<T> fb) {
   typedef typename C2<T>::t T2;
   constexpr int N = 4096;
+  constexpr bool GEN = VAR < 0;
   __shared__ T sbuf[FFT4096_LDS];
   const FftPlane<T> &pl = fb.p[blockIdx.y];
   const int row = blockIdx.x;
   const int64_t off = (int64_t)row * N;
   const T2 *phin = (const T2 *)fb.phase_in, *phout = (const T2 *)fb.phase_out;
+  const bool in_c = GEN ? pl.in_cplx != 0 : (VAR & 1) != 0;
+  const bool out_c = GEN ? pl.out_cplx != 0 : (VAR & 2) != 0;
+  const bool use_pin = GEN ? phin != nullptr : (VAR & 4) != 0;
+  const bool use_pout = GEN ? phout != nullptr : (VAR & 8) != 0;
+  const bool use_tab = GEN ? pl.table != nullptr : (VAR & 16) != 0;
   T2 v[16];
 #pragma unroll
   for (int j = 0; j < 16; j++) {
@@ -91,9 +102,9 @@ __global__ void __launch_bounds__(256) k_fft_rows4096(FftBatch<T> fb) {
     T2 a;
     a.x = 0;
     a.y = 0;
-    if (pl.in_cplx) a = ((const T2 *)pl.in)[off + x];
+    if (in_c) a = ((const T2 *)pl.in)[off + x];
     else a.x = pl.in[off + x];
-    if (phin) a = c_mul(a, phin[x]);
+    if (use_pin) a = c_mul(a, phin[x]);
     v[j] = a;
   }
   fft4096_reg<SIGN, T>(v, sbuf, (const T2 *)fb.tw);
@@ -103,15 +114,39 @@ __global__ void __launch_bounds__(256) k_fft_rows4096(FftBatch<T> fb) {
     T2 a = v[j];
     a.x *= fb.scale;
     a.y *= fb.scale;
-    if (phout) a = c_mul(a, phout[x]);
-    if (pl.table) {
+    if (use_pout) a = c_mul(a, phout[x]);
+    if (use_tab) {
       T tb = pl.table[off + x];
       a.x = tb * a.x;
       a.y = tb * a.y;
     }
-    if (pl.out_cplx) ((T2 *)pl.out)[off + x] = a;
+    if (out_c) ((T2 *)pl.out)[off + x] = a;
     else pl.out[off + x] = a.x;
   }
+}
+
+// One launch of the 4096-point kernel for planes of one form (VAR bits as
+// above); the forms the transforms use have their own kernels, others take
+// the form-generic one.
+template <typename T, int SIGN>
+static void launch_fft4096(const FftBatch<T> &fb, int nplanes, int var, hipStream_t st) {
+  const dim3 g(fb.nrows, nplanes);
+  if constexpr (SIGN < 0) {
+    switch (var) {
+    case 2 | 8: k_fft_rows4096<T, SIGN, 2 | 8><<<g, 256, 0, st>>>(fb); return;
+    case 1 | 2 | 8: k_fft_rows4096<T, SIGN, 1 | 2 | 8><<<g, 256, 0, st>>>(fb); return;
+    case 2 | 8 | 16: k_fft_rows4096<T, SIGN, 2 | 8 | 16><<<g, 256, 0, st>>>(fb); return;
+    case 1 | 2 | 8 | 16: k_fft_rows4096<T, SIGN, 1 | 2 | 8 | 16><<<g, 256, 0, st>>>(fb); return;
+    default: break;
+    }
+  } else {
+    switch (var) {
+    case 1 | 4: k_fft_rows4096<T, SIGN, 1 | 4><<<g, 256, 0, st>>>(fb); return;
+    case 1 | 2 | 4: k_fft_rows4096<T, SIGN, 1 | 2 | 4><<<g, 256, 0, st>>>(fb); return;
+    default: break;
+    }
+  }
+  k_fft_rows4096<T, SIGN, -1><<<g, 256, 0, st>>>(fb);
 }
 
 template <typename T, int SIGN>
@@ -126,9 +161,27 @@ static int launch_fft(const FftBatch<T> &fb, int nplanes, int N, hipStream_t st)
   }
     CASE(16) CASE(32) CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048)
 #undef CASE
-  case 4096:
-    k_fft_rows4096<T, SIGN><<<dim3(fb.nrows, nplanes), 256, 0, st>>>(fb);
+  case 4096: {
+    // planes grouped by form, one launch per form
+    const int pv = (fb.phase_in ? 4 : 0) | (fb.phase_out ? 8 : 0);
+    auto form = [&](int i) {
+      return (fb.p[i].in_cplx ? 1 : 0) | (fb.p[i].out_cplx ? 2 : 0) | pv | (fb.p[i].table ? 16 : 0);
+    };
+    bool done[MAXP] = {};
+    for (int i = 0; i < nplanes; i++) {
+      if (done[i]) continue;
+      const int var = form(i);
+      FftBatch<T> sub = fb;
+      int n = 0;
+      for (int j = i; j < nplanes; j++)
+        if (!done[j] && form(j) == var) {
+          sub.p[n++] = fb.p[j];
+          done[j] = true;
+        }
+      launch_fft4096<T, SIGN>(sub, n, var, st);
+    }
     break;
+  }
   default:
     return CHM_EUNSUPPORTED;
   }
