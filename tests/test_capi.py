@@ -64,6 +64,28 @@ def test_argument_errors_without_gpu():
         _lib.call("chm_push_x_f64", None, None, None, None, None, None, None, 0.1, -5, None)
 
 
+def test_align_multi_argument_errors_without_gpu():
+    """chm_align_multi checks its array table before it launches: array
+    counts outside 1..16, element sizes other than 4 and 8, a destination that
+    is also a source or another destination are refused (CHM_EINVAL); n = 0
+    launches nothing."""
+    from chimeracl_amd import _lib
+    L = _lib.lib
+    P = ctypes.c_void_p
+    src = (P * 2)(0x1000, 0x2000)
+    dst = (P * 2)(0x3000, 0x4000)
+    eb = (ctypes.c_int32 * 2)(8, 4)
+    a = lambda arr: ctypes.addressof(arr)
+    srt = 0x5000
+    assert L.chm_align_multi(srt, 10, 0, a(src), a(dst), a(eb), None) == -1
+    assert L.chm_align_multi(srt, 10, 17, a(src), a(dst), a(eb), None) == -1
+    assert L.chm_align_multi(srt, 10, 2, a(src), a(dst), a((ctypes.c_int32 * 2)(8, 3)), None) == -1
+    assert L.chm_align_multi(srt, 10, 2, a(src), a((P * 2)(0x3000, 0x1000)), a(eb), None) == -1
+    assert L.chm_align_multi(srt, 10, 2, a(src), a((P * 2)(0x3000, 0x3000)), a(eb), None) == -1
+    assert L.chm_align_multi(None, 10, 2, a(src), a(dst), a(eb), None) == -1
+    assert L.chm_align_multi(None, 0, 2, a(src), a(dst), a(eb), None) == 0
+
+
 def test_noalloc_sort_rejects_oversize_without_gpu():
     """The allocation-free sort checks the size against its fixed scratch
     before it touches the device: 2^22 particles on ~2^22 cells exceed it
