@@ -385,11 +385,12 @@ def kernel_table(sim, reps, w):
 
 
 def pmc_traffic(config, form, kernel="gather"):
-    """HBM bytes per launch of one of the roofline kernels (gather, deposit_J,
-    dht) on this workload, measured with rocprofv3 --pmc FETCH_SIZE /
-    WRITE_SIZE (copy-calibrated): read from the counter summaries in
-    profiles/ (files *_<config>_pmc.md, lines `traffic_bytes_<kernel>: {form:
-    bytes}`), or None."""
+    """(HBM bytes per launch, source file) of one of the roofline kernels
+    (gather, deposit_J, dht) on this workload: read from the rocprofv3
+    counter summaries committed in profiles/ (files *_<config>_pmc.md, lines
+    `traffic_bytes_<kernel>: {form: bytes}`; DRAM request counters for the
+    4096 x 512 workload, copy-calibrated FETCH_SIZE / WRITE_SIZE for the
+    2048 x 256 one), not measured in this run; (None, None) if absent."""
     import glob
     keys = [f"traffic_bytes_{kernel}:"] + (["traffic_bytes:"] if kernel == "gather" else [])
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"*_{config.lower()}_pmc.md"))):
@@ -399,10 +400,10 @@ def pmc_traffic(config, form, kernel="gather"):
                     if line.startswith(key):
                         v = json.loads(line.split(":", 1)[1]).get(form)
                         if v is not None:
-                            return v
+                            return v, os.path.relpath(path, ROOT)
         except OSError:
             continue
-    return None
+    return None, None
 
 
 def burst_time(fn, reps):
@@ -604,7 +605,9 @@ def dominant_roofline(sim, tm, w, config, form):
     if "dht" in tm:
         r["dht"] = dht_roofline(sim, tm["dht"], ident)
     for k in r:
-        r[k]["traffic"] = pmc_traffic(config, form, k)
+        r[k]["traffic"], src = pmc_traffic(config, form, k)
+        r[k]["traffic_source"] = (f"{src}: rocprofv3 counter run of the same kernels, "
+                                  "not measured in this run") if src else None
     top = max(r, key=lambda k: r[k]["ms_per_step"])
     out = dict(r[top])
     out["kernels"] = r
